@@ -1,0 +1,14 @@
+"""MI355X (gfx950) implementation of emida's EBSD pattern-offset pipeline.
+
+The compute path is libemida_amd.so (hand-written HIP kernels behind the C ABI in
+include/emida_amd.h).  This package is its Python host side: GpuOffset mirrors
+gpu_offset<T, IN>, get_offset mirrors get_offset<T>, kernels mirrors the
+kernel-level launchers.  Importing the package does not load the library; the
+first call does, and fails loudly if it was not built.
+"""
+from ._capi import CROSS_POLICY_BRUTE, CROSS_POLICY_FFT, EmidaError, LIB_PATH  # noqa: F401
+from .gpu_offset import (DeviceBuffer, GpuOffset, PinnedBuffer, data_index_dtype, device_query,  # noqa: F401
+                         get_offset, synchronize)
+
+__all__ = ["CROSS_POLICY_BRUTE", "CROSS_POLICY_FFT", "EmidaError", "GpuOffset", "DeviceBuffer", "PinnedBuffer",
+           "data_index_dtype", "get_offset", "synchronize", "device_query", "LIB_PATH"]

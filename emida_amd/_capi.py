@@ -1,0 +1,130 @@
+"""ctypes binding of include/emida_amd.h (libemida_amd.so, built in-tree)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libemida_amd.so")
+
+F32, F64, I32, U16 = 0, 1, 2, 3
+CROSS_POLICY_BRUTE, CROSS_POLICY_FFT = 0, 1
+OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED, ERR_NOMEM = range(6)
+
+_TYPE_OF = {np.dtype(np.float32): F32, np.dtype(np.float64): F64, np.dtype(np.int32): I32,
+            np.dtype(np.uint16): U16}
+_DTYPE_OF = {v: k for k, v in _TYPE_OF.items()}
+
+
+def type_code(dtype) -> int:
+    try:
+        return _TYPE_OF[np.dtype(dtype)]
+    except KeyError:
+        raise ValueError(f"unsupported element type {dtype}") from None
+
+
+def dtype_of(code: int):
+    return _DTYPE_OF[code]
+
+
+class EmidaError(RuntimeError):
+    """A failing library call (the reference throws std::runtime_error)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"emida error {code}: {msg}")
+        self.code = code
+
+
+class Size2(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_uint32), ("y", ctypes.c_uint32)]
+
+
+class OffsetInfo(ctypes.Structure):
+    _fields_ = [("total_slices", ctypes.c_uint32), ("neigh_size", ctypes.c_uint32),
+                ("data_index_bytes", ctypes.c_uint32), ("cross_size", Size2), ("corr_buffer", Size2),
+                ("fft_size", ctypes.c_uint32), ("device_bytes", ctypes.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library. Fails loudly when it has not been built: there is no
+    fallback implementation."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C emida_amd/csrc)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u32, i32, sz, u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_size_t, ctypes.c_uint64
+        dp = ctypes.POINTER(ctypes.c_double)
+        S = Size2
+        sigs = {
+            "emida_last_error": ([], ctypes.c_char_p),
+            "emida_offset_create": ([ctypes.POINTER(vp), i32, i32, S, ctypes.POINTER(S), u32, S, S, u32, i32, i32,
+                                     u32], i32),
+            "emida_offset_destroy": ([vp], None),
+            "emida_offset_get_info": ([vp, ctypes.POINTER(OffsetInfo)], i32),
+            "emida_offset_allocate_memory": ([vp, vp], i32),
+            "emida_offset_transfer_pic_to_device_async": ([vp, vp, ctypes.POINTER(u32)], i32),
+            "emida_offset_wait_uploads": ([vp], i32),
+            "emida_offset_get_offset_core": ([vp, vp, vp, u32], i32),
+            "emida_offset_finalize": ([vp, vp, vp, dp, dp], i32),
+            "emida_offset_get_offset": ([vp, vp, dp, dp], i32),
+            "emida_offset_process_device": ([vp, vp, u32, vp, vp, vp, vp, vp], i32),
+            "emida_offset_set_timing": ([vp, i32], i32),
+            "emida_offset_get_timing": ([vp, ctypes.POINTER(ctypes.c_char_p), dp, ctypes.POINTER(u64)], i32),
+            "emida_host_alloc": ([ctypes.POINTER(vp), sz], i32),
+            "emida_host_free": ([vp], i32),
+            "emida_device_alloc": ([ctypes.POINTER(vp), sz], i32),
+            "emida_device_free": ([vp], i32),
+            "emida_memcpy_h2d": ([vp, vp, sz], i32),
+            "emida_memcpy_d2h": ([vp, vp, sz], i32),
+            "emida_memset_device": ([vp, i32, sz], i32),
+            "emida_device_synchronize": ([], i32),
+            "emida_device_query": ([ctypes.c_char_p, sz, ctypes.POINTER(i32)], i32),
+            "emida_run_sum": ([i32, vp, vp, vp, S, S, u32, u32, vp], i32),
+            "emida_run_prepare_pics": ([i32, i32, i32, vp, vp, vp, vp, vp, vp, S, S, S, u32, u32, vp], i32),
+            "emida_run_cross_corr": ([i32, vp, vp, vp, S, S, u32, u32, vp], i32),
+            "emida_run_cross_corr_fft": ([i32, vp, vp, vp, S, u32, u32, vp], i32),
+            "emida_run_hadamard": ([i32, vp, vp, S, u32, u32, vp], i32),
+            "emida_run_maxarg_reduce": ([i32, vp, vp, S, u32, vp], i32),
+            "emida_run_extract_neighbors": ([i32, i32, vp, sz, vp, vp, i32, S, u32, vp], i32),
+            "emida_run_subpixel_max": ([i32, vp, i32, u32, vp, vp, vp], i32),
+            "emida_lstsq_matrix": ([i32, dp], i32),
+            "emida_generate_hanning": ([i32, sz, vp], i32),
+        }
+        for name, (args, res) in sigs.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != OK:
+        msg = lib().emida_last_error().decode(errors="replace")
+        raise EmidaError(rc, msg)
+
+
+def size2(v) -> Size2:
+    x, y = v
+    return Size2(int(x), int(y))
+
+
+def ptr(a):
+    """Address of a numpy array (or None)."""
+    if a is None:
+        return None
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def exported_symbols():
+    """Every function name declared in include/emida_amd.h that the library must export."""
+    hdr = os.path.join(os.path.dirname(_HERE), "include", "emida_amd.h")
+    import re
+    names = re.findall(r"^\s*(?:int|void|const char\*)\s+(emida_\w+)\s*\(", open(hdr).read(), re.M)
+    return names

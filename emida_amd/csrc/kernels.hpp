@@ -1,0 +1,68 @@
+// Host-side launchers of the gfx950 kernels (internal to the library).
+#pragma once
+
+#include "common.hpp"
+
+namespace emida_amd {
+
+// ---- prep.hip
+template <class IN, class S>
+void launch_sum(const IN* pics, S* sums, const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total,
+                hipStream_t st);
+template <class IN, class T, class S>
+void launch_prepare(const IN* pics, T* slices, const T* hx, const T* hy, const S* sums, const size2* begins,
+                    size2 src, size2 slice, size2 out, uint32_t nb, uint32_t total, hipStream_t st);
+
+// ---- brute.hip
+// res[(i*ref_slices + j)][...] = correlation of pics slice (i, j) with ref slice j.
+template <class T>
+void launch_cross_corr(const T* pics, const T* ref, T* res, size2 size, size2 res_size, uint32_t ref_slices,
+                       uint32_t batch, hipStream_t st);
+
+// ---- peak.hip
+// Offsets of the least-squares tables for fitsize 3, 5, 7, 9 in the device table.
+int lstsq_offset(int fit);
+const double* lstsq_device_table();  // uploads once per device
+void lstsq_host(int s, double* out);  // exact (A^T A)^-1 A^T, 6 x s^2
+
+// Per slice: arg-max over `area` values (lowest index on ties), F x F gather
+// with the reference's clamp and position policy, GPU least-squares fit and the
+// finalize formula.  Any output pointer may be null.
+template <class T>
+void launch_peak(const T* corr, size_t corr_len, uint32_t area, size2 cross, int policy, int fit,
+                 data_index<T>* maxes, T* neigh, double* offsets, double* coefs, uint32_t total, hipStream_t st);
+template <class T>
+void launch_maxarg(const T* data, uint32_t area, data_index<T>* out, uint32_t total, hipStream_t st);
+template <class T>
+void launch_extract(const T* data, size_t data_len, const data_index<T>* max_i, T* neigh, int s, size2 size,
+                    int policy, uint32_t total, hipStream_t st);
+// Fit only (subpixel_max_serial); with `maxes` non-null also the finalize formula
+// (offsets become pattern offsets instead of neighbourhood positions).
+template <class T>
+void launch_subpixel(const T* neigh, int s, uint32_t n, const data_index<T>* maxes, size2 cross, int policy,
+                     double* offsets, double* coefs, hipStream_t st);
+
+// ---- fft.hip
+// Largest padded FFT side handled by the single-kernel FFT pipeline.
+int fft_max_side(bool is_double);
+// Reference spectra: prepared reference slices -> half spectra in the layout the
+// pipeline kernel reads ((M/2+1) columns x M bit-reversed rows per ROI).
+template <class T, class IN, class S>
+void launch_fft_ref(const IN* pics, const S* sums, const T* hx, const T* hy, const size2* begins, size2 src,
+                    uint32_t side, uint32_t nb, cplx<T>* spectra, hipStream_t st);
+// Whole FFT pipeline of `total` slices (patterns x ROIs): prepare, R2C, multiply by
+// conj(reference spectrum), C2R, arg-max, gather, fit.  corr_out (optional)
+// receives the (2s)^2 circular correlation buffers.
+template <class T, class IN, class S>
+void launch_fft_pipe(const IN* pics, const S* sums, const T* hx, const T* hy, const size2* begins, size2 src,
+                     uint32_t side, uint32_t nb, uint32_t total, const cplx<T>* spectra, int fit,
+                     data_index<T>* maxes, T* neigh, double* offsets, double* coefs, T* corr_out, hipStream_t st);
+// Prepared slices in, circular correlation buffers out (cross_corr_fft seam).
+template <class T>
+void launch_fft_corr_prepared(const T* pics, const T* ref, T* res, uint32_t side, uint32_t ref_slices,
+                              uint32_t batch, hipStream_t st);
+template <class T>
+void launch_hadamard(cplx<T>* pics, const cplx<T>* ref, size2 one, uint32_t ref_slices, uint32_t batch,
+                     hipStream_t st);
+
+}  // namespace emida_amd

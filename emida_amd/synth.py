@@ -1,0 +1,141 @@
+"""Synthetic EBSD-like patterns and ROI grids for tests and benchmarks.
+
+Band model: a constant background of 2000 counts plus N Kikuchi-like bands
+(direction U[0, pi), signed distance of the band centre line from the image centre
+U over the image half-diagonal, Gaussian cross profile sigma U[6, 20] px, amplitude
+U[500, 3000]) multiplied by a radial falloff exp(-r^2 / (2 * 200^2)) around the
+image centre, then Poisson noise.  A deformed pattern resamples the noise-free
+reference bilinearly under x' = (I + eps)(x - c) + c + t (c = image centre), then
+draws fresh Poisson noise.  Content at reference position x therefore appears at
+x' in the deformed pattern; the local displacement at a ROI centre p is
+eps (p - c) + t and the offset the pipeline reports is its negative.
+"""
+from __future__ import annotations
+
+import hashlib
+import math
+
+import numpy as np
+
+try:
+    from scipy import ndimage as _ndi
+except Exception:  # pragma: no cover - scipy is present in this image
+    _ndi = None
+
+
+def seed_of(*parts) -> int:
+    """Deterministic 63-bit seed from any printable parts (config name, index...)."""
+    h = hashlib.sha256("/".join(str(p) for p in parts).encode()).digest()
+    return int.from_bytes(h[:8], "little") & ((1 << 63) - 1)
+
+
+def band_reference(size, n_bands=24, seed=0, background=2000.0, falloff=200.0):
+    """Noise-free reference pattern (float64, shape (H, W))."""
+    w, h = (size, size) if np.isscalar(size) else size
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    cx, cy = (w - 1) / 2.0, (h - 1) / 2.0
+    x, y = xx - cx, yy - cy
+    half_diag = 0.5 * math.hypot(w, h)
+    bands = np.zeros((h, w))
+    for _ in range(n_bands):
+        th = rng.uniform(0, math.pi)
+        rho = rng.uniform(-half_diag, half_diag)
+        sigma = rng.uniform(6, 20)
+        amp = rng.uniform(500, 3000)
+        d = x * math.cos(th) + y * math.sin(th) - rho
+        bands += amp * np.exp(-d * d / (2 * sigma * sigma))
+    return background + bands * np.exp(-(x * x + y * y) / (2 * falloff * falloff))
+
+
+def deform(clean, eps, t):
+    """Bilinear resampling of `clean` under x' = (I + eps)(x - c) + c + t."""
+    h, w = clean.shape
+    a = np.eye(2) + np.asarray(eps, np.float64).reshape(2, 2)
+    ainv = np.linalg.inv(a)
+    cx, cy = (w - 1) / 2.0, (h - 1) / 2.0
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    u = xx - cx - t[0]
+    v = yy - cy - t[1]
+    sx = ainv[0, 0] * u + ainv[0, 1] * v + cx
+    sy = ainv[1, 0] * u + ainv[1, 1] * v + cy
+    if _ndi is not None:
+        return _ndi.map_coordinates(clean, [sy, sx], order=1, mode="nearest")
+    sx = np.clip(sx, 0, w - 1)
+    sy = np.clip(sy, 0, h - 1)
+    x0 = np.minimum(np.floor(sx).astype(int), w - 2)
+    y0 = np.minimum(np.floor(sy).astype(int), h - 2)
+    fx, fy = sx - x0, sy - y0
+    return ((1 - fy) * ((1 - fx) * clean[y0, x0] + fx * clean[y0, x0 + 1]) +
+            fy * ((1 - fx) * clean[y0 + 1, x0] + fx * clean[y0 + 1, x0 + 1]))
+
+
+def poisson_u16(img, seed):
+    rng = np.random.default_rng(seed)
+    return np.clip(rng.poisson(np.maximum(img, 0)), 0, 65535).astype(np.uint16)
+
+
+def grid_centres(extent, k, radius):
+    """linspace(radius, extent - radius, k) rounded half-down."""
+    c = np.linspace(radius, extent - radius, k)
+    return np.ceil(c - 0.5).astype(int)
+
+
+def grid_begins(size, k, radius):
+    """ROI corners (x, y) of a k x k grid of 2r x 2r ROIs, row-major, x fastest."""
+    w, h = (size, size) if np.isscalar(size) else size
+    xs = grid_centres(w, k, radius) - radius
+    ys = grid_centres(h, k, radius) - radius
+    return [(int(x), int(y)) for y in ys for x in xs]
+
+
+def displacement_at(points, eps, t, size):
+    """eps (p - c) + t at ROI centres p ((n,2) array of x, y)."""
+    w, h = (size, size) if np.isscalar(size) else size
+    c = np.array([(w - 1) / 2.0, (h - 1) / 2.0])
+    p = np.asarray(points, np.float64) - c
+    return p @ np.asarray(eps, np.float64).reshape(2, 2).T + np.asarray(t, np.float64)
+
+
+class PatternSet:
+    """A reference pattern and deformed patterns with their deformation parameters."""
+
+    def __init__(self, name, size, n_patterns, eps_sigma=1e-3, t_range=2.0, n_bands=24, seed=0, unique=None):
+        self.name = name
+        self.size = (size, size) if np.isscalar(size) else tuple(size)
+        self.clean = band_reference(self.size, n_bands, seed_of(name, seed, "ref"))
+        self.ref = poisson_u16(self.clean, seed_of(name, seed, "ref-noise"))
+        self.eps, self.t = [], []
+        unique = n_patterns if unique is None else min(unique, n_patterns)
+        w, h = self.size
+        base = np.empty((unique, h, w), np.uint16)
+        for i in range(unique):
+            rng = np.random.default_rng(seed_of(name, seed, "def", i))
+            eps = rng.normal(0, eps_sigma, (2, 2))
+            t = rng.uniform(-t_range, t_range, 2)
+            base[i] = poisson_u16(deform(self.clean, eps, t), seed_of(name, seed, "def-noise", i))
+            self.eps.append(eps)
+            self.t.append(t)
+        reps = -(-n_patterns // unique)
+        self.pics = np.tile(base, (reps, 1, 1))[:n_patterns] if reps > 1 else base
+        self.eps = [self.eps[i % unique] for i in range(n_patterns)]
+        self.t = [self.t[i % unique] for i in range(n_patterns)]
+
+    def expected_offsets(self, begins, slice_size):
+        """-(displacement) at every ROI centre, (n_patterns * n_rois, 2)."""
+        sx, sy = (slice_size, slice_size) if np.isscalar(slice_size) else slice_size
+        centres = np.array([(x + sx / 2.0, y + sy / 2.0) for x, y in begins])
+        out = [-displacement_at(centres, e, t, self.size) for e, t in zip(self.eps, self.t)]
+        return np.concatenate(out)
+
+
+# The five workloads of the benchmark configurations (pattern side, patterns,
+# ROI grid k x k, ROI radius, policy, precision, fitsize, batch).
+CONFIGS = {
+    "brute_f64_256": dict(size=256, patterns=16, k=5, radius=16, policy="brute", dtype="f64", fit=3, batch=16,
+                          t_range=2.0),
+    "fft_f32_873": dict(size=873, patterns=256, k=20, radius=32, policy="fft", dtype="f32", fit=5, batch=64,
+                        t_range=3.0),
+    "brute_873_r64": dict(size=873, patterns=128, k=10, radius=64, policy="brute", dtype="f64", fit=7, batch=128,
+                          t_range=3.0),
+}

@@ -1,0 +1,217 @@
+/*
+ * emida_amd.h - C ABI of the MI355X (gfx950) pattern-offset library.
+ *
+ * One `emida_offset` object plays the role of the reference's
+ * `emida::gpu_offset<T, IN>` (src/gpu_offset.hpp:48-326): it holds the region of
+ * interest (ROI) layout, the prepared reference pattern and the device buffers
+ * of one batch shape, and runs the offset pipeline
+ *   slice sums -> prepare (mean, Hann) -> cross-correlation (brute | FFT)
+ *   -> arg-max -> F x F neighbourhood -> quadratic least-squares peak
+ * on the GPU.
+ *
+ * Conventions
+ *  - Every function returns EMIDA_OK (0) or an error code; emida_last_error()
+ *    gives the message of the last failure on the calling thread (the reference
+ *    prints and throws std::runtime_error, src/host_helpers.hpp:14-25; the C++
+ *    wrapper include/emida/gpu_offset.hpp rethrows).
+ *  - Sizes are {x = columns, y = rows}; images are row-major.
+ *  - "host" pointers may be pageable or pinned (emida_host_alloc); "device"
+ *    pointers come from emida_device_alloc or any HIP allocation on the current
+ *    device.  `stream` arguments are hipStream_t values passed as void* (NULL
+ *    means the object's own compute stream, or the null stream for the
+ *    kernel-level functions).
+ *  - No call allocates or frees device memory except create/allocate_memory/
+ *    destroy and the explicit alloc/free helpers.
+ */
+#ifndef EMIDA_AMD_H
+#define EMIDA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ types */
+
+/* Element types. T (correlation precision) is F32 or F64; IN (pattern pixel
+ * type) is U16, F32 or F64; I32 is accepted by emida_run_cross_corr only. */
+typedef enum { EMIDA_F32 = 0, EMIDA_F64 = 1, EMIDA_I32 = 2, EMIDA_U16 = 3 } emida_type;
+
+/* cross_policy, src/common.hpp:141-145 */
+typedef enum { EMIDA_CROSS_POLICY_BRUTE = 0, EMIDA_CROSS_POLICY_FFT = 1 } emida_cross_policy;
+
+enum {
+    EMIDA_OK = 0,
+    EMIDA_ERR_ARG = 1,         /* invalid argument */
+    EMIDA_ERR_HIP = 2,         /* HIP runtime error */
+    EMIDA_ERR_STATE = 3,       /* call order (e.g. compute before allocate_memory) */
+    EMIDA_ERR_UNSUPPORTED = 4, /* valid in the reference but not implemented here */
+    EMIDA_ERR_NOMEM = 5        /* device or pinned allocation failed */
+};
+
+/* size2_t, src/common.hpp:132 */
+typedef struct {
+    uint32_t x, y;
+} emida_size2;
+
+typedef struct emida_offset_s emida_offset;
+
+/* Derived sizes of an object (src/gpu_offset.hpp:100-125). */
+typedef struct {
+    uint32_t total_slices;     /* ROIs x batch */
+    uint32_t neigh_size;       /* fitsize^2 x total_slices */
+    uint32_t data_index_bytes; /* sizeof(data_index<T>): 8 (F32) or 16 (F64) */
+    emida_size2 cross_size;    /* effective correlation size (FFT forces 2s-1) */
+    emida_size2 corr_buffer;   /* arg-max domain: cross_size (brute) or 2s (FFT) */
+    uint32_t fft_size;         /* padded FFT side M (0 for brute) */
+    uint64_t device_bytes;     /* device memory held by the object */
+} emida_offset_info;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* emida_last_error(void);
+
+/* ------------------------------------------------------------------ drop-in object */
+
+/* gpu_offset(src_size, begins, slice_size, cross_size, batch_size, policy, s,
+ * in_buffers_size), src/gpu_offset.hpp:100-125.  `begins` is copied (the
+ * reference borrows it).  fitsize in {3,5,7,9}; in_buffers >= 1. */
+int emida_offset_create(emida_offset** out, emida_type T, emida_type IN, emida_size2 src_size,
+                        const emida_size2* begins, uint32_t n_begins, emida_size2 slice_size,
+                        emida_size2 cross_size, uint32_t batch_size, emida_cross_policy policy, int fitsize,
+                        uint32_t in_buffers);
+
+/* Frees every device and pinned buffer of the object (the reference never frees). */
+void emida_offset_destroy(emida_offset* h);
+
+int emida_offset_get_info(const emida_offset* h, emida_offset_info* info);
+
+/* allocate_memory(IN* temp), src/gpu_offset.hpp:127-211: allocates the buffers and
+ * prepares the reference pattern `temp` (host, src_size pixels) once: slices for
+ * BRUTE, spectra for FFT.  temp may be NULL (buffers only). */
+int emida_offset_allocate_memory(emida_offset* h, const void* temp);
+
+/* transfer_pic_to_device_async(IN* pic), src/gpu_offset.hpp:213-218: starts the
+ * upload of batch_size patterns (host) into the next input buffer (round robin,
+ * thread safe) and returns its index.  The upload waits for the previous
+ * computation that read the same buffer. */
+int emida_offset_transfer_pic_to_device_async(emida_offset* h, const void* pic, uint32_t* buffer_index);
+
+/* cudaStreamSynchronize(in_stream) as called by the reference's pipeline
+ * (src/process_files.hpp:196). */
+int emida_offset_wait_uploads(emida_offset* h);
+
+/* get_offset_core(maxes_i, neighbors, buffer), src/gpu_offset.hpp:234-278: runs the
+ * pipeline on input buffer `buffer_index` and copies the arg-max records
+ * (data_index<T>[total_slices]) and neighbourhoods (T[neigh_size]) to the host
+ * pointers, asynchronously on the output stream.  The subpixel fit also runs on
+ * the GPU; emida_offset_finalize returns its result. */
+int emida_offset_get_offset_core(emida_offset* h, void* maxes_i, void* neighbors, uint32_t buffer_index);
+
+/* finalize(maxes_i, neighbors), src/gpu_offset.hpp:280-301: waits for the output
+ * stream and writes offsets (double[2 x total_slices], x then y) and fit
+ * coefficients (double[6 x total_slices], a..f of a + bx + cy + dx^2 + exy + fy^2).
+ * For buffers filled by get_offset_core it returns the GPU fit of that call; for
+ * any other buffers it uploads them and fits them on the GPU. */
+int emida_offset_finalize(emida_offset* h, const void* maxes_i, const void* neighbors, double* offsets,
+                          double* coefs);
+
+/* get_offset(IN* pic), src/gpu_offset.hpp:220-231: synchronous upload, compute and
+ * finalize of one batch.  coefs may be NULL. */
+int emida_offset_get_offset(emida_offset* h, const void* pic, double* offsets, double* coefs);
+
+/* Device-resident form of get_offset_core + finalize (no reference counterpart;
+ * used by pipelines that keep patterns in HBM): `n_patterns` (1..batch_size)
+ * patterns at device pointer d_pics; per-ROI results to device pointers (any may
+ * be NULL), stream-ordered on `stream`. */
+int emida_offset_process_device(emida_offset* h, const void* d_pics, uint32_t n_patterns, void* d_maxes,
+                                void* d_neighbors, double* d_offsets, double* d_coefs, void* stream);
+
+/* Kernel timing with HIP events on the stream the kernels run on.  Returns the
+ * summed time and launch count of the dominant kernel (the FFT pipeline kernel
+ * for FFT, the correlation kernel for brute) since timing was enabled. */
+int emida_offset_set_timing(emida_offset* h, int enable);
+int emida_offset_get_timing(emida_offset* h, const char** kernel_name, double* total_ms, uint64_t* launches);
+
+/* ------------------------------------------------------------------ memory helpers */
+
+/* cuda_malloc_host (src/host_helpers.hpp:74-80): pinned host memory. */
+int emida_host_alloc(void** ptr, size_t bytes);
+int emida_host_free(void* ptr);
+int emida_device_alloc(void** ptr, size_t bytes);
+int emida_device_free(void* ptr);
+int emida_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int emida_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int emida_memset_device(void* dst, int value, size_t bytes);
+int emida_device_synchronize(void);
+/* Device name and compute-unit count of the current device. */
+int emida_device_query(char* name, size_t name_len, int* cu_count);
+
+/* ------------------------------------------------------------------ kernel-level seam
+ * Counterparts of the launchers in src/kernels.cuh:56-125 (the reference's tests call
+ * them directly).  All pointers are device pointers. */
+
+/* run_sum(data, sums, begins, src_size, slice_size, begins_size, batch_size),
+ * src/kernels.cuh:117-119, src/sum.cu:160-167.  Sum type: uint32 for U16 input,
+ * else IN. */
+int emida_run_sum(emida_type IN, const void* data, void* sums, const emida_size2* begins, emida_size2 src_size,
+                  emida_size2 slice_size, uint32_t begins_size, uint32_t batch_size, void* stream);
+
+/* run_prepare_pics, src/kernels.cuh:93-104, src/prepare_pics.cu:46-63.  S is the sum
+ * element type (U16 means uint32 sums). */
+int emida_run_prepare_pics(emida_type IN, emida_type T, emida_type S, const void* pic, void* slices,
+                           const void* hanning_x, const void* hanning_y, const void* sums,
+                           const emida_size2* begins, emida_size2 src_size, emida_size2 slice_size,
+                           emida_size2 out_size, uint32_t begins_size, uint32_t batch_size, void* stream);
+
+/* run_cross_corr(pics, ref, res, size, res_size, ref_slices, batch_size),
+ * src/kernels.cuh:55-62, src/cross_corr.cu:68-74.  T in {F32, F64, I32};
+ * res_size <= 2*size+1 per axis. */
+int emida_run_cross_corr(emida_type T, const void* pics, const void* ref, void* res, emida_size2 size,
+                         emida_size2 res_size, uint32_t ref_slices, uint32_t batch_size, void* stream);
+
+/* cross_corr_fft(), src/gpu_offset.hpp:303-316 (R2C, hadamard, C2R; unnormalised):
+ * prepared size.x x size.y slices -> (2 size.x) x (2 size.y) circular correlation
+ * buffers, batch x ref_slices of them.  size must be square with a power-of-two
+ * side <= 64. */
+int emida_run_cross_corr_fft(emida_type T, const void* pics, const void* ref, void* res, emida_size2 size,
+                             uint32_t ref_slices, uint32_t batch_size, void* stream);
+
+/* run_hadamard(pics, ref, one_size, ref_slices, batch_size), src/kernels.cuh:122-123,
+ * src/hadamard.cu:43-49: pics[i] = pics[i] * conj(ref[i mod (one_size.area * ref_slices)])
+ * on interleaved complex T. */
+int emida_run_hadamard(emida_type T, void* pics, const void* ref, emida_size2 one_size, uint32_t ref_slices,
+                       uint32_t batch_size, void* stream);
+
+/* run_maxarg_reduce(data, maxes, maxarg, size, block_size, batch_size),
+ * src/kernels.cuh:106-107, src/maxarg.cu:180-187: data_index<T> of the largest
+ * value per slice of size.x*size.y values (lowest index on ties). */
+int emida_run_maxarg_reduce(emida_type T, const void* data, void* maxarg, emida_size2 size, uint32_t batch_size,
+                            void* stream);
+
+/* run_extract_neighbors<T, pos_policy>(data, max_i, neighbors, s, src_size,
+ * batch_size), src/kernels.cuh:109-110, src/extract_neighbors.cu:43-49.
+ * pos_policy BRUTE = cross_res_pos_policy_id, FFT = cross_res_pos_policy_fft.
+ * Reads at or past data_len elements (possible through the one-past-the-edge
+ * clamp of the last slice) return 0. */
+int emida_run_extract_neighbors(emida_type T, emida_cross_policy pos_policy, const void* data, size_t data_len,
+                                const void* max_i, void* neighbors, int s, emida_size2 src_size,
+                                uint32_t batch_size, void* stream);
+
+/* subpixel_max_serial(pic, s, batch_size), src/subpixel_max.hpp:76-97, on the GPU:
+ * offsets double[2n] (x, y within the s x s neighbourhood), coefs double[6n]. */
+int emida_run_subpixel_max(emida_type T, const void* neighbors, int s, uint32_t n, double* offsets,
+                           double* coefs, void* stream);
+
+/* get_lstsq_matrix(s), src/subpixel_max.cpp:5-20: (A^T A)^-1 A^T as 6 x s^2 doubles. */
+int emida_lstsq_matrix(int s, double* out);
+
+/* generate_hanning<T>(size), src/host_helpers.hpp:29-37. */
+int emida_generate_hanning(emida_type T, size_t size, void* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMIDA_AMD_H */

@@ -1,0 +1,80 @@
+"""CPU-only checks of the HIP library: it loads, exports every declared entry point,
+and its host-side pieces (tables, windows, argument validation) agree with the
+oracle.  No GPU call is made here."""
+import numpy as np
+import pytest
+
+import emida_amd
+from emida_amd import _capi as C
+from emida_amd import kernels as K
+from emida_amd import synth
+from oracle import oracle as O
+
+
+def test_library_exports_every_declared_symbol():
+    L = C.lib()
+    names = C.exported_symbols()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("s", [3, 5, 7, 9])
+def test_lstsq_table_matches_oracle_bitwise(s):
+    np.testing.assert_array_equal(K.lstsq_matrix(s), O.lstsq_matrix(s))
+
+
+@pytest.mark.parametrize("n", [2, 5, 32, 64, 128, 101])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_hanning_matches_oracle_bitwise(n, dt):
+    np.testing.assert_array_equal(K.generate_hanning(n, dt), O.hann(n, dt))
+
+
+def _create(**kw):
+    args = dict(T=np.float64, IN=np.uint16, src_size=(128, 128), begins=[(0, 0)], slice_size=(32, 32),
+                cross_size=(63, 63), batch_size=1, policy=emida_amd.CROSS_POLICY_BRUTE, s=3)
+    args.update(kw)
+    return emida_amd.GpuOffset(**args)
+
+
+def test_create_valid_configurations_without_gpu():
+    g = _create()
+    assert g.total_slices == 1 and g.cross_size == (63, 63) and g.neigh_size == 9
+    g = _create(policy=emida_amd.CROSS_POLICY_FFT, cross_size=(5, 5), s=5, batch_size=3, begins=[(0, 0), (10, 20)])
+    # the FFT policy forces cross = 2s - 1 (src/gpu_offset.hpp:114)
+    assert g.cross_size == (63, 63) and g.corr_buffer == (64, 64) and g.fft_size == 64
+    assert g.total_slices == 6 and g.neigh_size == 150
+    assert g.data_index.itemsize == 16 and _create(T=np.float32).data_index.itemsize == 8
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(s=4), C.ERR_ARG),
+    (dict(begins=[(100, 0)]), C.ERR_ARG),               # ROI does not fit
+    (dict(cross_size=(64, 63)), C.ERR_ARG),             # > 2s-1
+    (dict(batch_size=0), C.ERR_ARG),
+    (dict(begins=[]), C.ERR_ARG),
+    (dict(IN=np.int32), C.ERR_ARG),
+    (dict(policy=emida_amd.CROSS_POLICY_FFT, slice_size=(48, 48)), C.ERR_UNSUPPORTED),
+    (dict(policy=emida_amd.CROSS_POLICY_FFT, slice_size=(32, 16)), C.ERR_UNSUPPORTED),
+    (dict(policy=emida_amd.CROSS_POLICY_FFT, slice_size=(128, 128)), C.ERR_UNSUPPORTED),
+])
+def test_create_rejects_invalid_arguments(kw, code):
+    with pytest.raises(emida_amd.EmidaError) as e:
+        _create(**kw)
+    assert e.value.code == code
+
+
+def test_synthetic_grid_matches_benchmark_definition():
+    # 5 x 5 centres on 256 px with radius 16 are exactly 16, 72, ..., 240
+    assert list(synth.grid_centres(256, 5, 16)) == [16, 72, 128, 184, 240]
+    assert list(synth.grid_centres(1024, 31, 32)) == list(range(32, 993, 32))
+    b = synth.grid_begins(873, 20, 32)
+    assert len(b) == 400 and all(0 <= x and x + 64 <= 873 and 0 <= y and y + 64 <= 873 for x, y in b)
+
+
+def test_synthetic_patterns_are_deterministic():
+    a = synth.PatternSet("det", 64, 2)
+    b = synth.PatternSet("det", 64, 2)
+    np.testing.assert_array_equal(a.pics, b.pics)
+    np.testing.assert_array_equal(a.ref, b.ref)
+    assert a.pics.dtype == np.uint16 and a.pics.shape == (2, 64, 64)

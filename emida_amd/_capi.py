@@ -82,6 +82,7 @@ def lib():
             "emida_device_free": ([vp], i32),
             "emida_memcpy_h2d": ([vp, vp, sz], i32),
             "emida_memcpy_d2h": ([vp, vp, sz], i32),
+            "emida_memcpy_d2d": ([vp, vp, sz], i32),
             "emida_memset_device": ([vp, i32, sz], i32),
             "emida_device_synchronize": ([], i32),
             "emida_device_query": ([ctypes.c_char_p, sz, ctypes.POINTER(i32)], i32),

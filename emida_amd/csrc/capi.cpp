@@ -12,7 +12,9 @@
 //  - ROI bounds and sizes are validated up front instead of reading out of bounds.
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -123,6 +125,8 @@ struct emida_offset_s {
     std::vector<hipEvent_t> ev_up, ev_done;
     std::atomic<uint32_t> rr{0};
     std::mutex upload_mu;
+    std::condition_variable upload_cv;
+    std::vector<char> pending;  // buffer holds an upload no get_offset_core consumed yet
 
     size2* d_begins = nullptr;
     void* d_hx = nullptr;
@@ -250,6 +254,7 @@ struct emida_offset_s {
             ev_up.push_back(a);
             ev_done.push_back(b);
         }
+        pending.assign(n_in, 0);
         d_begins = (size2*)alloc(begins.size() * sizeof(size2));
         EMIDA_HIP(hipMemcpy(d_begins, begins.data(), begins.size() * sizeof(size2), hipMemcpyHostToDevice));
         with_T(T, [&](auto t) {
@@ -361,6 +366,11 @@ struct emida_offset_s {
         if (o.used) EMIDA_HIP(hipStreamWaitEvent(compute, o.copied, 0));
         run(d_in[bi], batch, o.maxes, o.neigh, o.offsets, o.coefs, compute);
         EMIDA_HIP(hipEventRecord(ev_done[bi], compute));
+        {
+            std::lock_guard<std::mutex> lock(upload_mu);
+            pending[bi] = 0;
+        }
+        upload_cv.notify_all();
         EMIDA_HIP(hipStreamWaitEvent(out_stream, ev_done[bi], 0));
         const size_t nk = (size_t)fit * fit;
         if (maxes_i) EMIDA_HIP(hipMemcpyAsync(maxes_i, o.maxes, total * ti_bytes(), hipMemcpyDeviceToHost, out_stream));
@@ -378,18 +388,27 @@ struct emida_offset_s {
 
     void finalize(const void* maxes_i, const void* neighbors, double* offsets, double* coefs) {
         require_allocated();
-        EMIDA_HIP(hipStreamSynchronize(out_stream));
+        const slot* found = nullptr;
+        slot copy;
         {
             std::lock_guard<std::mutex> lock(slot_mu);
             for (auto& s : slots)
                 if (s.maxes == maxes_i && s.neigh == neighbors && s.stamp != 0) {
-                    EMIDA_HIP(hipEventSynchronize(s.done));
-                    if (offsets) std::memcpy(offsets, s.h_offsets, total * 2 * sizeof(double));
-                    if (coefs) std::memcpy(coefs, s.h_coefs, total * 6 * sizeof(double));
-                    return;
+                    copy = s;
+                    found = &copy;
+                    break;
                 }
         }
+        if (found) {
+            // wait outside the lock so the next get_offset_core is not held up; the
+            // caller reuses these result buffers only after this returns
+            EMIDA_HIP(hipEventSynchronize(found->done));
+            if (offsets) std::memcpy(offsets, found->h_offsets, total * 2 * sizeof(double));
+            if (coefs) std::memcpy(coefs, found->h_coefs, total * 6 * sizeof(double));
+            return;
+        }
         if (!maxes_i || !neighbors) throw arg_error("finalize: maxes_i and neighbors are required");
+        EMIDA_HIP(hipStreamSynchronize(out_stream));
         // foreign buffers: fit them on the GPU
         const size_t nk = (size_t)fit * fit;
         void* dm = dmalloc(total * ti_bytes());
@@ -510,8 +529,10 @@ int emida_offset_transfer_pic_to_device_async(emida_offset* h, const void* pic, 
     return guarded([&] {
         if (!h || !pic) throw arg_error("null argument");
         h->require_allocated();
-        std::lock_guard<std::mutex> lock(h->upload_mu);
+        std::unique_lock<std::mutex> lock(h->upload_mu);
         const uint32_t i = h->rr.fetch_add(1) % h->n_in;
+        h->upload_cv.wait(lock, [&] { return !h->pending[i]; });
+        h->pending[i] = 1;
         EMIDA_HIP(hipStreamWaitEvent(h->in_stream, h->ev_done[i], 0));
         EMIDA_HIP(hipMemcpyAsync(h->d_in[i], pic, (size_t)h->src.area() * h->batch * h->in_bytes(),
                                  hipMemcpyHostToDevice, h->in_stream));
@@ -623,6 +644,9 @@ int emida_memcpy_h2d(void* dst, const void* src, size_t bytes) {
 }
 int emida_memcpy_d2h(void* dst, const void* src, size_t bytes) {
     return guarded([&] { EMIDA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+int emida_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+    return guarded([&] { EMIDA_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice)); });
 }
 int emida_memset_device(void* dst, int value, size_t bytes) {
     return guarded([&] { EMIDA_HIP(hipMemset(dst, value, bytes)); });

@@ -1,0 +1,197 @@
+// emida/gpu_offset.hpp - C++17 drop-in for the reference's emida::gpu_offset<T, IN>
+// and emida::get_offset<T> (src/gpu_offset.hpp:48-326, src/get_offset.hpp:8-20),
+// implemented over the C ABI in emida_amd.h.  Link with -lemida_amd.
+//
+// Same constructor and member signatures; errors throw std::runtime_error after
+// printing to stderr, like the reference's CUCH (src/host_helpers.hpp:14-25).
+// Differences: `begins` is copied (the reference keeps the pointer); device
+// memory is released by the destructor; uploads are awaited with
+// wait_for_uploads() instead of cudaStreamSynchronize(in_stream).
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "emida_amd.h"
+
+namespace emida {
+
+using esize_t = uint32_t;  // src/common.hpp:24
+
+template <typename T>
+struct data_index {  // src/common.hpp:26-31
+    T data;
+    esize_t index;
+};
+
+template <typename T>
+inline bool operator==(const data_index<T>& a, const data_index<T>& b) {
+    return a.data == b.data && a.index == b.index;
+}
+
+template <typename T>
+struct vec2 {  // src/common.hpp:41-123 (the members used by callers)
+    T x;
+    T y;
+    T area() const { return x * y; }
+    vec2 operator+(const vec2& r) const { return {x + r.x, y + r.y}; }
+    vec2 operator-(const vec2& r) const { return {x - r.x, y - r.y}; }
+    template <typename U>
+    vec2 operator*(const U& r) const { return {x * r, y * r}; }
+    template <typename U>
+    vec2 operator-(const U& r) const { return {x - r, y - r}; }
+    esize_t pos(esize_t cols) const { return y * cols + x; }
+};
+
+template <typename T>
+inline bool operator==(const vec2<T>& a, const vec2<T>& b) {
+    return a.x == b.x && a.y == b.y;
+}
+
+using size2_t = vec2<esize_t>;
+
+enum cross_policy { CROSS_POLICY_BRUTE = EMIDA_CROSS_POLICY_BRUTE, CROSS_POLICY_FFT = EMIDA_CROSS_POLICY_FFT };
+
+template <typename T>
+struct offsets_t {  // src/subpixel_max.hpp:69-74
+    std::vector<vec2<T>> offsets;
+    std::vector<std::array<T, 6>> coefs;
+};
+
+namespace detail {
+
+inline void check(int rc, const char* what) {
+    if (rc != EMIDA_OK) {
+        std::string msg = std::string("emida error ") + std::to_string(rc) + " in " + what + ": " + emida_last_error();
+        std::cerr << msg << "\n";
+        throw std::runtime_error(msg);
+    }
+}
+
+template <typename T>
+constexpr emida_type type_code() {
+    if constexpr (std::is_same<T, float>::value) return EMIDA_F32;
+    else if constexpr (std::is_same<T, double>::value) return EMIDA_F64;
+    else if constexpr (std::is_same<T, uint16_t>::value) return EMIDA_U16;
+    else return EMIDA_I32;
+}
+
+inline emida_size2 c(size2_t s) { return {s.x, s.y}; }
+
+}  // namespace detail
+
+// gpu_offset<T, IN>: T in {float, double}; IN in {uint16_t, float, double}.
+template <typename T, typename IN>
+class gpu_offset {
+    emida_offset* h_ = nullptr;
+    esize_t total_ = 0, neigh_ = 0;
+
+    offsets_t<double> unpack(const std::vector<double>& o, const std::vector<double>& cf) const {
+        offsets_t<double> r;
+        r.offsets.resize(total_);
+        r.coefs.resize(total_);
+        for (esize_t i = 0; i < total_; ++i) {
+            r.offsets[i] = {o[2 * i], o[2 * i + 1]};
+            for (int k = 0; k < 6; ++k) r.coefs[i][k] = cf[6 * i + k];
+        }
+        return r;
+    }
+
+public:
+    gpu_offset() = default;
+    gpu_offset(size2_t src_size, const std::vector<size2_t>* begins, size2_t slice_size, size2_t cross_size,
+               esize_t batch_size, cross_policy policy = CROSS_POLICY_BRUTE, int s = 3, size_t in_buffers_size = 1) {
+        std::vector<emida_size2> b(begins->size());
+        for (size_t i = 0; i < b.size(); ++i) b[i] = detail::c((*begins)[i]);
+        detail::check(emida_offset_create(&h_, detail::type_code<T>(), detail::type_code<IN>(), detail::c(src_size),
+                                          b.data(), (uint32_t)b.size(), detail::c(slice_size), detail::c(cross_size),
+                                          batch_size, (emida_cross_policy)policy, s, (uint32_t)in_buffers_size),
+                      "gpu_offset");
+        emida_offset_info info;
+        detail::check(emida_offset_get_info(h_, &info), "gpu_offset");
+        total_ = info.total_slices;
+        neigh_ = info.neigh_size;
+    }
+    gpu_offset(const gpu_offset&) = delete;
+    gpu_offset& operator=(const gpu_offset&) = delete;
+    gpu_offset(gpu_offset&& o) noexcept : h_(o.h_), total_(o.total_), neigh_(o.neigh_) { o.h_ = nullptr; }
+    gpu_offset& operator=(gpu_offset&& o) noexcept {
+        std::swap(h_, o.h_);
+        total_ = o.total_;
+        neigh_ = o.neigh_;
+        return *this;
+    }
+    ~gpu_offset() {
+        if (h_) emida_offset_destroy(h_);
+    }
+
+    esize_t total_slices() const { return total_; }
+    esize_t neigh_size() const { return neigh_; }
+
+    // src/gpu_offset.hpp:127
+    void allocate_memory(IN* temp) { detail::check(emida_offset_allocate_memory(h_, temp), "allocate_memory"); }
+
+    // src/gpu_offset.hpp:213
+    esize_t transfer_pic_to_device_async(IN* pic) {
+        uint32_t i = 0;
+        detail::check(emida_offset_transfer_pic_to_device_async(h_, pic, &i), "transfer_pic_to_device_async");
+        return i;
+    }
+
+    // cudaStreamSynchronize(in_stream), src/process_files.hpp:196
+    void wait_for_uploads() { detail::check(emida_offset_wait_uploads(h_), "wait_for_uploads"); }
+
+    // src/gpu_offset.hpp:220
+    offsets_t<double> get_offset(IN* pic) {
+        std::vector<double> o(2 * (size_t)total_), cf(6 * (size_t)total_);
+        detail::check(emida_offset_get_offset(h_, pic, o.data(), cf.data()), "get_offset");
+        return unpack(o, cf);
+    }
+
+    // src/gpu_offset.hpp:234
+    void get_offset_core(data_index<T>* maxes_i, T* neighbors, size_t cu_pic_index) {
+        detail::check(emida_offset_get_offset_core(h_, maxes_i, neighbors, (uint32_t)cu_pic_index), "get_offset_core");
+    }
+
+    // src/gpu_offset.hpp:280
+    offsets_t<double> finalize(data_index<T>* maxes_i, T* neighbors) {
+        std::vector<double> o(2 * (size_t)total_), cf(6 * (size_t)total_);
+        detail::check(emida_offset_finalize(h_, maxes_i, neighbors, o.data(), cf.data()), "finalize");
+        return unpack(o, cf);
+    }
+
+    emida_offset* handle() { return h_; }
+};
+
+// get_offset<T>(pic, temp, size, cross_size, policy), src/get_offset.hpp:8-14: one ROI
+// covering the whole size.x x size.y picture at {0, 0}, batch 1.
+template <typename T>
+inline std::vector<vec2<double>> get_offset(T* pic, T* temp, size2_t size, size2_t cross_size,
+                                            cross_policy c_policy = CROSS_POLICY_BRUTE) {
+    std::vector<size2_t> begins = {{0, 0}};
+    gpu_offset<T, T> offs(size, &begins, size, cross_size, 1, c_policy);
+    offs.allocate_memory(temp);
+    return offs.get_offset(pic).offsets;
+}
+
+// src/get_offset.hpp:16-20
+template <typename T>
+inline std::vector<vec2<double>> get_offset(T* pic, T* temp, size2_t size, size_t) {
+    return get_offset(pic, temp, size, size * 2 - 1);
+}
+
+// cuda_malloc_host (src/host_helpers.hpp:74-80)
+template <typename T>
+T* cuda_malloc_host(size_t num_elements) {
+    void* p = nullptr;
+    detail::check(emida_host_alloc(&p, num_elements * sizeof(T)), "host_alloc");
+    return (T*)p;
+}
+inline void cuda_free_host(void* p) { detail::check(emida_host_free(p), "host_free"); }
+
+}  // namespace emida

@@ -94,8 +94,12 @@ int emida_offset_allocate_memory(emida_offset* h, const void* temp);
 
 /* transfer_pic_to_device_async(IN* pic), src/gpu_offset.hpp:213-218: starts the
  * upload of batch_size patterns (host) into the next input buffer (round robin,
- * thread safe) and returns its index.  The upload waits for the previous
- * computation that read the same buffer. */
+ * thread safe) and returns its index.  The upload is ordered after the previous
+ * computation that read the same buffer; the call blocks while that buffer still
+ * holds an upload that no get_offset_core call has consumed (a single-threaded
+ * caller must not have more than in_buffers uploads outstanding).  `pic` must
+ * stay unchanged until emida_offset_wait_uploads (or the consuming computation)
+ * returns. */
 int emida_offset_transfer_pic_to_device_async(emida_offset* h, const void* pic, uint32_t* buffer_index);
 
 /* cudaStreamSynchronize(in_stream) as called by the reference's pipeline
@@ -143,6 +147,7 @@ int emida_device_alloc(void** ptr, size_t bytes);
 int emida_device_free(void* ptr);
 int emida_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int emida_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int emida_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int emida_memset_device(void* dst, int value, size_t bytes);
 int emida_device_synchronize(void);
 /* Device name and compute-unit count of the current device. */

@@ -1,4 +1,5 @@
-"""End-to-end goldens of the reference's get_offset tests, as data.
+"""End-to-end goldens of the reference's get_offset tests, as data
+(the 49 batched offsets live in get_offset_batched.txt, shared with the C++ test).
 
 Values are the expected offsets written in test/get_offset_test.cpp:25-26 (single
 64x64 ROI at {0,0}, brute, cross 127x127, tolerance 2e-14) and :143-194 (49 ROIs of
@@ -8,57 +9,10 @@ Values are the expected offsets written in test/get_offset_test.cpp:25-26 (singl
 BIGGER = (0.07583538046549165, -0.0982055210473689)
 BIGGER_TOL = 2e-14
 BATCHED_TOL = 7e-13
-BATCHED = [
-    (0.07583538046549165, -0.0982055210473689),
-    (0.0663767374671167, -0.04069601894434527),
-    (0.074197482004692, -0.030619500269573052),
-    (0.05436786376404257, -0.005589154658409257),
-    (0.06733198717111577, 0.0517236573394797),
-    (0.041554775305378655, 0.06618103013488508),
-    (0.08077034898616375, 0.06563583626893177),
-    (0.02731542373235385, -0.07039470582336094),
-    (0.06008171647840754, -0.02744745873626897),
-    (0.029170141077287326, -0.023289975910856242),
-    (0.0326187646798104, -0.005564197682964789),
-    (0.03391155180750616, 0.019564499299086435),
-    (0.04385983944938232, 0.05449764848280836),
-    (0.02024519655957846, 0.06849415625676158),
-    (0.01368431240032919, -0.07111342587600689),
-    (0.012048011945815063, -0.03926198717228857),
-    (0.013961762064383265, -0.015440254382809826),
-    (0.015755335565160067, -0.005555724262663375),
-    (0.023421560567179256, 0.009032311534404869),
-    (0.023248682942536902, 0.0350821811421298),
-    (0.03053119272782112, 0.06845563900186846),
-    (-0.0032150199066265372, -0.07778718128383844),
-    (-0.005324141702296004, -0.04152366763450033),
-    (0.005280972355535596, -0.01804919638136937),
-    (-0.0019341378339703397, -0.006479061775870321),
-    (0.0008047285811727534, 0.017339600500093866),
-    (-0.003395056024281473, 0.05269941701205738),
-    (-0.011578085777955494, 0.05729089241229701),
-    (-0.005359767640584323, -0.05472352384999368),
-    (-0.05244171495544947, -0.04443498940228352),
-    (-0.01161102567282768, -0.017353308868052864),
-    (-0.008499046745086503, -0.004588709518344558),
-    (-0.01571490670305309, 0.007374566094114243),
-    (-0.007446791064488423, 0.039194860149095234),
-    (-0.036628348106169994, 0.08157786777366738),
-    (-0.023866359887549038, -0.051201391631913395),
-    (-0.046419482310795956, -0.06510193845873857),
-    (-0.047653378663540025, -0.03436734987603529),
-    (-0.05175330219534402, -0.008762911732411283),
-    (-0.05304557884009853, 0.02318941866170121),
-    (-0.05468521432916873, 0.07303630737680322),
-    (-0.0748218633099853, 0.08867438132940464),
-    (-0.05947110619956675, -0.058893164709672874),
-    (-0.04432777844017011, -0.04082147832301786),
-    (-0.05327469916555572, -0.03227153665071825),
-    (-0.06626366960730223, 0.013074685592769697),
-    (-0.06791710237803272, 0.040805467203661294),
-    (-0.047683181620783444, 0.03318911716604589),
-    (-0.11304959759046085, 0.061363430916259176),
-]
+BATCHED = [tuple(float(v) for v in line.split())
+           for line in open(__import__("os").path.join(__import__("os").path.dirname(__file__),
+                                                       "get_offset_batched.txt"))
+           if line.strip() and not line.startswith("#")]
 
 
 def slice_begins(src=256, size=64, step=32):

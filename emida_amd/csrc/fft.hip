@@ -1,64 +1,171 @@
-// Single-kernel FFT cross-correlation pipeline (LDS resident, one workgroup per
-// ROI of one pattern).
+// Single-kernel FFT cross-correlation pipeline (one workgroup per ROI of one
+// pattern, everything resident in LDS and registers).
 //
 // Reference: src/gpu_offset.hpp:303-316 (R2C of the zero-padded 2s x 2s slice,
-// multiply by the conjugate reference spectrum, unnormalised C2R),
-// src/hadamard.cu:27-41 (the product p * conj(r)), src/maxarg.cu and
-// src/extract_neighbors.cu with cross_res_pos_policy_fft (src/kernels.cuh:36-53)
-// for the circular result layout, src/subpixel_max.hpp for the fit.  The reference
-// runs these as separate cuFFT plans and kernels with the correlation buffers in
-// global memory; here one workgroup keeps the whole 2-D transform of one ROI in
-// LDS and never writes the correlation to HBM.
+// product with the conjugate reference spectrum, unnormalised C2R),
+// src/hadamard.cu:27-41 (p * conj(r)), src/maxarg.cu and src/extract_neighbors.cu with
+// cross_res_pos_policy_fft (src/kernels.cuh:36-53) for the circular result layout,
+// src/subpixel_max.hpp for the fit.  The reference runs separate cuFFT plans and
+// kernels with every intermediate in global memory; here nothing but the inputs, the
+// reference spectra and the per-ROI results touch HBM.
 //
-// Transform layout (M = 2s, power of two, S = M/2 = slice side, NC = M/2 + 1):
-//  1. rows: rows 2p and 2p+1 are packed as one complex row (re, im) and
-//     transformed with an in-place radix-2 decimation-in-frequency FFT whose first
-//     pass is pruned (the upper half of every row is zero padding).  The rows live
-//     in the still unused upper halves of the spectrum columns.
-//  2. split: the two real-row spectra are separated from the packed spectrum
-//     (X[k] and conj(X[M-k])) into columns kx = 0..M/2 of the half spectrum.
-//  3. columns: DIF FFT over y (first pass pruned), result in bit-reversed row
-//     order; the reference spectrum is stored in the same order, so the product
-//     needs no reordering.  The last DIF pass, the product and the first pass of
-//     the inverse decimation-in-time FFT run on the same register pair.
-//  4. inverse rows: rows 2q and 2q+1 of the column result become one complex
-//     sequence Ga + i Gb (Hermitian extension of each half spectrum, imaginary
-//     parts of the DC and Nyquist terms dropped as a C2R does); one inverse FFT
-//     yields both real output rows.
-//  5. arg-max over the (2s)^2 circular buffer, F x F gather through the FFT
-//     position policy, double-precision least-squares fit and finalize.
+// FFT scheme.  A length-M transform (M = 2s) is computed by a group of L lanes of
+// one wave, each lane holding R = M/L values: an R-point DFT in registers over
+// n = L*n1 + n2 (lane n2), a twiddle W_M^(n2*k1), one transpose through the group's
+// LDS scratch, and L-point DFTs in registers.  Lane j then holds X[k] for
+// k = (j*Q + i) + R*k2 (Q = R/L, i < Q, k2 < L) - the "frequency layout".  The
+// inverse starts from that layout and ends in natural order, so the product with the
+// reference spectrum (stored in the same layout) needs no reordering.
+//
+// Stages for one ROI (S = M/2 = slice side, NC = S + 1 half-spectrum columns):
+//  0. load the S x S ROI, integer sum (uint16) or given sum, prepare
+//     ((px - mean) * hx * hy, the reference's order) into LDS;
+//  A. rows 2p and 2p+1 packed as one complex row: S/2 forward transforms;
+//  B. column kx of the half spectrum, split from the packed rows
+//     (X[k] and conj(X[M-k])); columns 0 and S are real and packed into one
+//     complex column; forward transform, product with conj(reference), inverse;
+//  C. rows 2q and 2q+1 of the column result packed as Ga + i Gb (Hermitian
+//     extension, DC and Nyquist imaginary parts dropped as a C2R does), inverse
+//     transform yields both real correlation rows;
+//  D. arg-max over the (2s)^2 circular buffer, F x F gather through the FFT
+//     position policy, double-precision least-squares fit, finalize.
 #include <type_traits>
 
 #include "kernels.hpp"
 
 namespace emida_amd {
 
+// ------------------------------------------------------------------ complex helpers
 template <class T>
-__device__ inline cplx<T> cadd(cplx<T> a, cplx<T> b) {
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) {
     return {a.re + b.re, a.im + b.im};
 }
 template <class T>
-__device__ inline cplx<T> csub(cplx<T> a, cplx<T> b) {
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) {
     return {a.re - b.re, a.im - b.im};
 }
 template <class T>
-__device__ inline cplx<T> cmul(cplx<T> a, cplx<T> w) {
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> w) {
     return {a.re * w.re - a.im * w.im, a.re * w.im + a.im * w.re};
 }
 template <class T>
-__device__ inline cplx<T> cmul_conj(cplx<T> a, cplx<T> w) {  // a * conj(w)
+__device__ __forceinline__ cplx<T> cmul_conj(cplx<T> a, cplx<T> w) {  // a * conj(w)
     return {a.re * w.re + a.im * w.im, a.im * w.re - a.re * w.im};
 }
+// The reference's product p * conj(r) (src/hadamard.cu:39-40), same expression.
+template <class T>
+__device__ __forceinline__ cplx<T> hadamard(cplx<T> p, cplx<T> r) {
+    return {p.re * r.re + p.im * r.im, -p.re * r.im + p.im * r.re};
+}
 
-template <int M>
-__device__ inline uint32_t bitrev(uint32_t k) {
-    constexpr int L = __builtin_ctz(M);
-    return __brev(k) >> (32 - L);
+// exp(-2 pi i j / 16), j < 8: correctly rounded cos/sin of multiples of pi/8.
+constexpr double kC1 = 0.9238795325112867, kS1 = 0.3826834323650898, kR2 = 0.7071067811865476;
+
+// v * W_16^j (conjugate twiddle when INV); j is a compile-time constant after unrolling.
+template <class T, bool INV>
+__device__ __forceinline__ cplx<T> twiddle16(cplx<T> v, int j) {
+    const T sg = INV ? T(-1) : T(1);
+    switch (j) {
+        case 0: return v;
+        case 4: return INV ? cplx<T>{-v.im, v.re} : cplx<T>{v.im, -v.re};
+        case 2: return cmul(v, cplx<T>{T(kR2), sg * T(-kR2)});
+        case 6: return cmul(v, cplx<T>{T(-kR2), sg * T(-kR2)});
+        case 1: return cmul(v, cplx<T>{T(kC1), sg * T(-kS1)});
+        case 3: return cmul(v, cplx<T>{T(kS1), sg * T(-kC1)});
+        case 5: return cmul(v, cplx<T>{T(-kS1), sg * T(-kC1)});
+        default: return cmul(v, cplx<T>{T(-kC1), sg * T(-kS1)});  // 7
+    }
+}
+
+__host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+__host__ __device__ constexpr int brev_c(int v, int bits) {
+    int r = 0;
+    for (int b = 0; b < bits; ++b) r |= ((v >> b) & 1) << (bits - 1 - b);
+    return r;
+}
+
+// In-register DFT of N <= 16 points, natural order in and out (radix-2
+// decimation in frequency; the final bit reversal only renames registers).
+template <class T, int N, bool INV>
+__device__ __forceinline__ void dft_reg(cplx<T> (&x)[N]) {
+    static_assert(N >= 1 && N <= 16 && (N & (N - 1)) == 0, "N must be a power of two <= 16");
+#pragma unroll
+    for (int len = N; len >= 2; len >>= 1) {
+        const int half = len / 2;
+#pragma unroll
+        for (int s0 = 0; s0 < N; s0 += len) {
+#pragma unroll
+            for (int k = 0; k < half; ++k) {
+                const cplx<T> a = x[s0 + k], b = x[s0 + k + half];
+                x[s0 + k] = cadd(a, b);
+                x[s0 + k + half] = twiddle16<T, INV>(csub(a, b), k * (16 / len));
+            }
+        }
+    }
+    cplx<T> y[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) y[i] = x[brev_c(i, ilog2(N))];
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = y[i];
+}
+
+// Intra-wave LDS hand-off between the lanes of one FFT group.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Forward transform: lane n2 holds x[n1] = input[L*n1 + n2]; on return lane j holds
+// X[(j*Q + i) + R*k2] in x[i*L + k2].  scr: L*(R+1) elements of this group.
+template <class T, int R, int L, int M>
+__device__ __forceinline__ void fft_fwd(cplx<T> (&x)[R], int lane, cplx<T>* scr, const cplx<T>* tw) {
+    constexpr int Q = R / L;
+    dft_reg<T, R, false>(x);
+#pragma unroll
+    for (int k1 = 1; k1 < R; ++k1) x[k1] = cmul(x[k1], tw[lane * k1]);
+#pragma unroll
+    for (int k1 = 0; k1 < R; ++k1) scr[lane * (R + 1) + k1] = x[k1];
+    wave_sync();
+    cplx<T> y[Q][L];
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+#pragma unroll
+        for (int m = 0; m < L; ++m) y[i][m] = scr[m * (R + 1) + lane * Q + i];
+    wave_sync();
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        dft_reg<T, L, false>(y[i]);
+#pragma unroll
+        for (int k2 = 0; k2 < L; ++k2) x[i * L + k2] = y[i][k2];
+    }
+}
+
+// Inverse (unnormalised) transform from the frequency layout to natural order:
+// on return lane n2 holds output[L*n1 + n2] in x[n1].
+template <class T, int R, int L, int M>
+__device__ __forceinline__ void fft_inv(cplx<T> (&x)[R], int lane, cplx<T>* scr, const cplx<T>* tw) {
+    constexpr int Q = R / L;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        cplx<T> y[L];
+#pragma unroll
+        for (int k2 = 0; k2 < L; ++k2) y[k2] = x[i * L + k2];
+        dft_reg<T, L, true>(y);
+        const int k1 = lane * Q + i;
+#pragma unroll
+        for (int n2 = 0; n2 < L; ++n2) scr[n2 * (R + 1) + k1] = n2 == 0 ? y[0] : cmul_conj(y[n2], tw[n2 * k1]);
+    }
+    wave_sync();
+#pragma unroll
+    for (int k1 = 0; k1 < R; ++k1) x[k1] = scr[lane * (R + 1) + k1];
+    wave_sync();
+    dft_reg<T, R, true>(x);
 }
 
 // Prepared pixel (src/prepare_pics.cu:36-41), no contraction.
 template <class T, class IN>
-__device__ inline T prep_px(IN px, T mean, T wx, T wy) {
+__device__ __forceinline__ T prep_px(IN px, T mean, T wx, T wy) {
 #pragma clang fp contract(off)
     T v = (T)px;
     v -= mean;
@@ -67,22 +174,22 @@ __device__ inline T prep_px(IN px, T mean, T wx, T wy) {
 }
 
 template <class T>
-__device__ inline T mean_of(T sum, uint32_t area) {
+__device__ __forceinline__ T mean_of(T sum, uint32_t area) {
 #pragma clang fp contract(off)
     return sum / (T)area;
 }
 
 template <class T, class IN, class SUM>
 struct fft_args {
-    const IN* pics;        // patterns (FROM_PATTERN)
-    const SUM* sums;       // per-slice sums (FROM_PATTERN, IN != uint16)
-    const T* hx;           // Hann windows (FROM_PATTERN)
+    const IN* pics;          // patterns (FROM_PATTERN)
+    const SUM* sums;         // per-slice sums (FROM_PATTERN, IN != uint16)
+    const T* hx;             // Hann windows (FROM_PATTERN)
     const T* hy;
-    const size2* begins;   // ROI corners
-    size2 src;             // pattern size
-    uint32_t nb;           // ROIs per pattern
-    uint32_t total;        // slices in this launch
-    const T* prepared;     // prepared slices (!FROM_PATTERN), s x s each
+    const size2* begins;     // ROI corners
+    size2 src;               // pattern size
+    uint32_t nb;             // ROIs per pattern
+    uint32_t total;          // slices in this launch
+    const T* prepared;       // prepared s x s slices (!FROM_PATTERN)
     const cplx<T>* spectra;  // reference spectra (PIPE)
     cplx<T>* spectra_out;    // reference spectra (!PIPE)
     int fit;
@@ -91,32 +198,38 @@ struct fft_args {
     T* neigh;
     double* offsets;
     double* coefs;
-    T* corr_out;           // optional (2s)^2 circular buffers
+    T* corr_out;             // optional (2s)^2 circular buffers, row-major
 };
 
+// Geometry of one transform size.
 template <class T, int M>
-struct fft_geom {
-    static constexpr int S = M / 2;
-    static constexpr int NC = M / 2 + 1;
-    static constexpr int CS = M + 1;  // column stride in complex elements
-    static constexpr size_t lds_bytes = (size_t)(NC * CS + M / 2) * sizeof(cplx<T>);
-    // row stage work area (packed row pair p, position x) in the upper column halves
-    __device__ static inline int rw(int p, int x) {
-        const int t = p * M + x;
-        return (t / S) * CS + S + (t % S);
-    }
-    // inverse row work area (row pair q, position x)
-    __device__ static inline int iw(int q, int x) { return (x >> 1) * CS + 2 * q + (x & 1); }
+struct fft_plan {
+    static constexpr int S = M / 2;                  // slice side
+    static constexpr int NC = S + 1;                 // half-spectrum columns
+    static constexpr int L = M >= 64 ? 8 : (M >= 16 ? 4 : 2);  // lanes per transform
+    static constexpr int R = M / L;                  // values per lane
+    static constexpr int Q = R / L;
+    static_assert(Q >= 1 && R * L == M, "bad transform split");
+    static constexpr int NT = S * L < 64 ? 64 : S * L;
+    // LDS slot stride (complex elements): holds a column of M values or one group's
+    // transpose scratch; congruent to 8 mod 16 so the groups of a wave spread over
+    // all banks.
+    static constexpr int need = (L * (R + 1) > M ? L * (R + 1) : M);
+    static constexpr int CS = ((need + 7) / 16) * 16 + 8;
+    static constexpr int P_OFF = NC * CS - S * S / 2;  // prepared ROI (S*S reals) at the end
+    static_assert(P_OFF >= (S / 2) * CS, "prepared ROI overlaps the row stage");
+    static constexpr size_t lds_bytes = (size_t)(NC * CS + M) * sizeof(cplx<T>);
 };
 
-template <class T, class IN, class SUM, int M, int NT, bool FROM_PATTERN, bool PIPE>
-__global__ __launch_bounds__(NT) void fft_kernel(fft_args<T, IN, SUM> a) {
-    using G = fft_geom<T, M>;
+template <class T, class IN, class SUM, int M, bool FROM_PATTERN, bool PIPE>
+__global__ __launch_bounds__((fft_plan<T, M>::NT)) void fft_kernel(fft_args<T, IN, SUM> a) {
+    using P = fft_plan<T, M>;
     using C = cplx<T>;
-    constexpr int S = G::S, NC = G::NC, CS = G::CS;
+    constexpr int S = P::S, L = P::L, R = P::R, Q = P::Q, CS = P::CS, NT = P::NT;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C* B = reinterpret_cast<C*>(lds_raw);
-    C* tw = B + NC * CS;
+    C* tw = B + P::NC * CS;
+    T* pre = reinterpret_cast<T*>(B + P::P_OFF);
     __shared__ uint32_t red_u[NT / kWave];
     __shared__ T sv[NT / kWave];
     __shared__ uint32_t si[NT / kWave];
@@ -127,22 +240,31 @@ __global__ __launch_bounds__(NT) void fft_kernel(fft_args<T, IN, SUM> a) {
     if (sn >= a.total) return;
     const uint32_t bn = sn % a.nb;
     const int tid = threadIdx.x;
+    const int lane = tid % L, grp = tid / L;
 
-    // twiddles W_M^j = exp(-2 pi i j / M), j < M/2, from double precision
-    for (int j = tid; j < M / 2; j += NT) {
+    // W_M^t for t < M, from double precision
+    for (int t = tid; t < M; t += NT) {
         double s, c;
-        sincospi(2.0 * j / M, &s, &c);
-        tw[j] = C{(T)c, (T)(-s)};
+        sincospi(2.0 * t / M, &s, &c);
+        tw[t] = C{(T)c, (T)(-s)};
     }
 
-    // ---- 0. input rows (prepared values), packed pairs, pruned first DIF pass
+    // ---- 0. prepared ROI into LDS
     if constexpr (FROM_PATTERN) {
         const size2 b0 = a.begins[bn];
         const IN* pic = a.pics + (size_t)(sn / a.nb) * a.src.area() + (size_t)b0.y * a.src.x + b0.x;
+        constexpr int E = (S * S + NT - 1) / NT;
+        IN px[E];
+#pragma unroll
+        for (int it = 0; it < E; ++it) {
+            const int e = tid + it * NT;
+            px[it] = e < S * S ? pic[(size_t)(e / S) * a.src.x + (e % S)] : IN(0);
+        }
         T mean;
         if constexpr (std::is_same<IN, uint16_t>::value) {
             uint32_t su = 0;
-            for (int e = tid; e < S * S; e += NT) su += pic[(size_t)(e / S) * a.src.x + (e % S)];
+#pragma unroll
+            for (int it = 0; it < E; ++it) su += px[it];
 #pragma unroll
             for (int off = kWave / 2; off > 0; off >>= 1) su += __shfl_xor(su, off);
             if (tid % kWave == 0) red_u[tid / kWave] = su;
@@ -154,194 +276,186 @@ __global__ __launch_bounds__(NT) void fft_kernel(fft_args<T, IN, SUM> a) {
         } else {
             mean = mean_of<T>((T)a.sums[sn], S * S);
         }
-        __syncthreads();  // twiddles visible
-        for (int e = tid; e < (S / 2) * S; e += NT) {
-            const int p = e / S, x = e % S;
-            const T wx = a.hx[x];
-            const T va = prep_px<T, IN>(pic[(size_t)(2 * p) * a.src.x + x], mean, wx, a.hy[2 * p]);
-            const T vb = prep_px<T, IN>(pic[(size_t)(2 * p + 1) * a.src.x + x], mean, wx, a.hy[2 * p + 1]);
-            const C z{va, vb};
-            B[G::rw(p, x)] = z;
-            B[G::rw(p, x + S)] = cmul(z, tw[x]);
+#pragma unroll
+        for (int it = 0; it < E; ++it) {
+            const int e = tid + it * NT;
+            if (e < S * S) pre[e] = prep_px<T, IN>(px[it], mean, a.hx[e % S], a.hy[e / S]);
         }
     } else {
-        __syncthreads();
         const T* src = a.prepared + (size_t)sn * S * S;
-        for (int e = tid; e < (S / 2) * S; e += NT) {
-            const int p = e / S, x = e % S;
-            const C z{src[(2 * p) * S + x], src[(2 * p + 1) * S + x]};
-            B[G::rw(p, x)] = z;
-            B[G::rw(p, x + S)] = cmul(z, tw[x]);
-        }
+        for (int e = tid; e < S * S; e += NT) pre[e] = src[e];
     }
     __syncthreads();
 
-    // ---- 1. row DIF passes, span S down to 2
+    // ---- A. packed row pairs, forward
+    if (grp < S / 2) {
+        const int p = grp;
+        C x[R];
 #pragma unroll
-    for (int len = S; len >= 2; len >>= 1) {
-        const int half = len / 2;
-        for (int b = tid; b < (S / 2) * (M / 2); b += NT) {
-            const int p = b / (M / 2), j = b % (M / 2);
-            const int k = j % half;
-            const int i0 = (j / half) * len + k, i1 = i0 + half;
-            const C w = tw[k * (M / len)];
-            const C x0 = B[G::rw(p, i0)], x1 = B[G::rw(p, i1)];
-            B[G::rw(p, i0)] = cadd(x0, x1);
-            B[G::rw(p, i1)] = cmul(csub(x0, x1), w);
+        for (int n1 = 0; n1 < R; ++n1) {
+            if (n1 < R / 2) {
+                const int n = L * n1 + lane;
+                x[n1] = C{pre[(2 * p) * S + n], pre[(2 * p + 1) * S + n]};
+            } else {
+                x[n1] = C{T(0), T(0)};
+            }
         }
-        __syncthreads();
-    }
-
-    // ---- 2. split packed spectra into half-spectrum columns
-    for (int e = tid; e < (S / 2) * NC; e += NT) {
-        const int p = e / NC, k = e % NC;
-        const C zk = B[G::rw(p, bitrev<M>(k))];
-        const C zm = B[G::rw(p, bitrev<M>((M - k) & (M - 1)))];
-        const C ra{(zk.re + zm.re) * (T)0.5, (zk.im - zm.im) * (T)0.5};
-        const C rb{(zk.im + zm.im) * (T)0.5, (zm.re - zk.re) * (T)0.5};
-        B[k * CS + 2 * p] = ra;
-        B[k * CS + 2 * p + 1] = rb;
-    }
-    __syncthreads();
-
-    // ---- 3. column DIF: pruned first pass, then spans S .. 4 (2 is fused below)
-    for (int e = tid; e < NC * S; e += NT) {
-        const int kx = e / S, y = e % S;
-        B[kx * CS + y + S] = cmul(B[kx * CS + y], tw[y]);
-    }
-    __syncthreads();
+        C* slot = B + p * CS;
+        fft_fwd<T, R, L, M>(x, lane, slot, tw);
 #pragma unroll
-    for (int len = S; len >= 4; len >>= 1) {
-        const int half = len / 2;
-        for (int b = tid; b < NC * (M / 2); b += NT) {
-            const int kx = b / (M / 2), j = b % (M / 2);
-            const int k = j % half;
-            const int i0 = kx * CS + (j / half) * len + k, i1 = i0 + half;
-            const C w = tw[k * (M / len)];
-            const C x0 = B[i0], x1 = B[i1];
-            B[i0] = cadd(x0, x1);
-            B[i1] = cmul(csub(x0, x1), w);
-        }
-        __syncthreads();
+        for (int i = 0; i < Q; ++i)
+#pragma unroll
+            for (int k2 = 0; k2 < L; ++k2) slot[(lane * Q + i) + R * k2] = x[i * L + k2];
     }
+    __syncthreads();
 
+    // ---- B. half-spectrum columns
+    C x[R];
+    const int c = grp;  // column job (0 = packed columns 0 and S)
+    if (c < S) {
+#pragma unroll
+        for (int n1 = 0; n1 < R; ++n1) {
+            if (n1 < R / 2) {
+                const int y = L * n1 + lane;
+                const C* row = B + (y >> 1) * CS;
+                if (c == 0) {
+                    const C z0 = row[0], zs = row[S];
+                    x[n1] = (y & 1) ? C{z0.im, zs.im} : C{z0.re, zs.re};
+                } else {
+                    const C zk = row[c], zm = row[M - c];
+                    x[n1] = (y & 1) ? C{(zk.im + zm.im) * T(0.5), (zm.re - zk.re) * T(0.5)}
+                                    : C{(zk.re + zm.re) * T(0.5), (zk.im - zm.im) * T(0.5)};
+                }
+            } else {
+                x[n1] = C{T(0), T(0)};
+            }
+        }
+    }
+    __syncthreads();  // row spectra consumed; slots become column scratch
+    if (c < S) {
+        C* slot = B + c * CS;
+        fft_fwd<T, R, L, M>(x, lane, slot, tw);
+        if (c == 0) {
+            // split the packed real columns: P0[k] = (Z[k] + conj(Z[-k]))/2,
+            // PS[k] = (Z[k] - conj(Z[-k]))/(2i)
+#pragma unroll
+            for (int i = 0; i < Q; ++i)
+#pragma unroll
+                for (int k2 = 0; k2 < L; ++k2) slot[(lane * Q + i) + R * k2] = x[i * L + k2];
+            wave_sync();
+            C p0[R], ps[R];
+#pragma unroll
+            for (int i = 0; i < Q; ++i)
+#pragma unroll
+                for (int k2 = 0; k2 < L; ++k2) {
+                    const int k = (lane * Q + i) + R * k2;
+                    const C z = x[i * L + k2], zm = slot[(M - k) & (M - 1)];
+                    p0[i * L + k2] = C{(z.re + zm.re) * T(0.5), (z.im - zm.im) * T(0.5)};
+                    ps[i * L + k2] = C{(z.im + zm.im) * T(0.5), (zm.re - z.re) * T(0.5)};
+                }
+            wave_sync();
+            if constexpr (PIPE) {
+                const C* r0 = a.spectra + ((size_t)bn * P::NC + 0) * M + lane * R;
+                const C* rs = a.spectra + ((size_t)bn * P::NC + S) * M + lane * R;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const C h0 = hadamard(p0[r], r0[r]), hs = hadamard(ps[r], rs[r]);
+                    x[r] = C{h0.re - hs.im, h0.im + hs.re};  // H0 + i HS
+                }
+            } else {
+                C* o0 = a.spectra_out + ((size_t)bn * P::NC + 0) * M + lane * R;
+                C* os = a.spectra_out + ((size_t)bn * P::NC + S) * M + lane * R;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    o0[r] = p0[r];
+                    os[r] = ps[r];
+                }
+            }
+        } else {
+            if constexpr (PIPE) {
+                const C* rc = a.spectra + ((size_t)bn * P::NC + c) * M + lane * R;
+#pragma unroll
+                for (int r = 0; r < R; ++r) x[r] = hadamard(x[r], rc[r]);
+            } else {
+                C* oc = a.spectra_out + ((size_t)bn * P::NC + c) * M + lane * R;
+#pragma unroll
+                for (int r = 0; r < R; ++r) oc[r] = x[r];
+            }
+        }
+        if constexpr (PIPE) {
+            fft_inv<T, R, L, M>(x, lane, slot, tw);
+            if (c == 0) {
+                C* slot_s = B + S * CS;
+#pragma unroll
+                for (int n1 = 0; n1 < R; ++n1) {
+                    const int y = L * n1 + lane;
+                    slot[y] = C{x[n1].re, T(0)};
+                    slot_s[y] = C{x[n1].im, T(0)};
+                }
+            } else {
+#pragma unroll
+                for (int n1 = 0; n1 < R; ++n1) slot[L * n1 + lane] = x[n1];
+            }
+        }
+    }
     if constexpr (!PIPE) {
-        // last DIF pass and store of the reference spectrum (bit-reversed rows)
-        C* out = a.spectra_out + (size_t)bn * NC * M;
-        for (int b = tid; b < NC * (M / 2); b += NT) {
-            const int kx = b / (M / 2), j = b % (M / 2);
-            const C x0 = B[kx * CS + 2 * j], x1 = B[kx * CS + 2 * j + 1];
-            out[kx * M + 2 * j] = cadd(x0, x1);
-            out[kx * M + 2 * j + 1] = csub(x0, x1);
-        }
         return;
     } else {
-        // ---- 4. last DIF pass, product with conj(reference), first DIT pass
-        const C* ref = a.spectra + (size_t)bn * NC * M;
-        for (int b = tid; b < NC * (M / 2); b += NT) {
-            const int kx = b / (M / 2), j = b % (M / 2);
-            const int i0 = kx * CS + 2 * j;
-            const C x0 = B[i0], x1 = B[i0 + 1];
-            const C r0 = ref[kx * M + 2 * j], r1 = ref[kx * M + 2 * j + 1];
-            const C h0 = cmul_conj(cadd(x0, x1), r0);
-            const C h1 = cmul_conj(csub(x0, x1), r1);
-            B[i0] = cadd(h0, h1);
-            B[i0 + 1] = csub(h0, h1);
-        }
         __syncthreads();
 
-        // ---- 5. column inverse DIT, spans 4 .. M
+        // ---- C. packed output row pairs, inverse
+        const int q = grp;
+        if (q < S) {
 #pragma unroll
-        for (int len = 4; len <= M; len <<= 1) {
-            const int half = len / 2;
-            for (int b = tid; b < NC * (M / 2); b += NT) {
-                const int kx = b / (M / 2), j = b % (M / 2);
-                const int k = j % half;
-                const int i0 = kx * CS + (j / half) * len + k, i1 = i0 + half;
-                const C w = tw[k * (M / len)];
-                const C x0 = B[i0];
-                const C x1 = cmul_conj(B[i1], w);
-                B[i0] = cadd(x0, x1);
-                B[i1] = csub(x0, x1);
-            }
-            __syncthreads();
-        }
-
-        // ---- 6. inverse rows: Z = Ga + i Gb in bit-reversed order
-        constexpr int E = (S * M + NT - 1) / NT;
-        C zr[E];
+            for (int i = 0; i < Q; ++i)
 #pragma unroll
-        for (int it = 0; it < E; ++it) {
-            const int e = tid + it * NT;
-            if (e < S * M) {
-                const int k = e / S, q = e % S;
-                C ga, gb;
-                if (k <= S) {
-                    ga = B[k * CS + 2 * q];
-                    gb = B[k * CS + 2 * q + 1];
-                } else {
-                    const C ca = B[(M - k) * CS + 2 * q], cb = B[(M - k) * CS + 2 * q + 1];
-                    ga = C{ca.re, -ca.im};
-                    gb = C{cb.re, -cb.im};
+                for (int k2 = 0; k2 < L; ++k2) {
+                    const int k = (lane * Q + i) + R * k2;
+                    const bool mirror = k > S;
+                    const C* col = B + (mirror ? M - k : k) * CS;
+                    C ga = col[2 * q], gb = col[2 * q + 1];
+                    if (mirror) {
+                        ga.im = -ga.im;
+                        gb.im = -gb.im;
+                    }
+                    if (k == 0 || k == S) {
+                        ga.im = T(0);
+                        gb.im = T(0);
+                    }
+                    x[i * L + k2] = C{ga.re - gb.im, ga.im + gb.re};
                 }
-                if (k == 0 || k == S) {
-                    ga.im = 0;
-                    gb.im = 0;
-                }
-                zr[it] = C{ga.re - gb.im, ga.im + gb.re};
-            }
         }
-        __syncthreads();
-#pragma unroll
-        for (int it = 0; it < E; ++it) {
-            const int e = tid + it * NT;
-            if (e < S * M) {
-                const int k = e / S, q = e % S;
-                B[G::iw(q, bitrev<M>(k))] = zr[it];
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int len = 2; len <= M; len <<= 1) {
-            const int half = len / 2;
-            for (int b = tid; b < S * (M / 2); b += NT) {
-                const int q = b % S, j = b / S;
-                const int k = j % half;
-                const int x0i = (j / half) * len + k, x1i = x0i + half;
-                const C w = tw[k * (M / len)];
-                const C x0 = B[G::iw(q, x0i)];
-                const C x1 = cmul_conj(B[G::iw(q, x1i)], w);
-                B[G::iw(q, x0i)] = cadd(x0, x1);
-                B[G::iw(q, x1i)] = csub(x0, x1);
-            }
-            __syncthreads();
-        }
-
-        // ---- 7. arg-max over the circular (2s)^2 buffer (lowest index on ties)
-        T best = -std::numeric_limits<T>::infinity();
+        __syncthreads();  // column results consumed; slots become row scratch
+        T best = -(T)__builtin_huge_val();
         uint32_t bi = 0xffffffffu;
-        T* cout = a.corr_out ? a.corr_out + (size_t)sn * M * M : nullptr;
-        for (int idx = tid; idx < M * M; idx += NT) {
-            const int y = idx / M, x = idx % M;
-            const C z = B[G::iw(y >> 1, x)];
-            const T v = (y & 1) ? z.im : z.re;
-            if (cout) cout[idx] = v;
-            if (v > best) {
-                best = v;
-                bi = (uint32_t)idx;
+        if (q < S) {
+            C* slot = B + q * CS;
+            fft_inv<T, R, L, M>(x, lane, slot, tw);
+            T* cout = a.corr_out ? a.corr_out + (size_t)sn * M * M : nullptr;
+#pragma unroll
+            for (int n1 = 0; n1 < R; ++n1) {
+                const int xx = L * n1 + lane;
+                slot[xx] = x[n1];
+                const uint32_t i0 = (uint32_t)(2 * q) * M + xx, i1 = i0 + M;
+                argmax_combine(best, bi, x[n1].re, i0);
+                argmax_combine(best, bi, x[n1].im, i1);
+                if (cout) {
+                    cout[i0] = x[n1].re;
+                    cout[i1] = x[n1].im;
+                }
             }
         }
         block_argmax<T, NT>(best, bi, sv, si);
 
-        // ---- 8. gather (FFT position policy, clamp to [0, 2s-1]) and fit
+        // ---- D. gather (FFT position policy, clamp to [0, 2s-1]) and fit
         const int fit = a.fit, kk = fit * fit, r = (fit - 1) / 2;
         const size2 cross{(uint32_t)M - 1, (uint32_t)M - 1};
         const size2 mp = shift_pos(POLICY_FFT, bi, cross);
         for (int t = tid; t < kk; t += NT) {
             const int i = t % fit, j = t / fit;
-            const size2 c = clamp_pos((int)mp.x - r + i, (int)mp.y - r + j, cross);
-            const int qx = (int)((c.x + S + 1) % M), qy = (int)((c.y + S + 1) % M);
-            const C z = B[G::iw(qy >> 1, qx)];
+            const size2 cc = clamp_pos((int)mp.x - r + i, (int)mp.y - r + j, cross);
+            const int qx = (int)((cc.x + S + 1) % M), qy = (int)((cc.y + S + 1) % M);
+            const C z = B[(qy >> 1) * CS + qx];
             const T v = (qy & 1) ? z.im : z.re;
             nbv[t] = v;
             if (a.neigh) a.neigh[(size_t)sn * kk + t] = v;
@@ -361,8 +475,8 @@ __global__ __launch_bounds__(NT) void fft_kernel(fft_args<T, IN, SUM> a) {
                 a.maxes[sn].data = best;
                 a.maxes[sn].index = bi;
             }
-            const double b = cf[1], c = cf[2], d = cf[3], e = cf[4], f = cf[5];
-            const double py = (b * e - 2 * c * d) / (4 * f * d - e * e);
+            const double b = cf[1], c2 = cf[2], d = cf[3], e = cf[4], f = cf[5];
+            const double py = (b * e - 2 * c2 * d) / (4 * f * d - e * e);
             const double px = (-e * py - b) / (2 * d);
             const double ox = -((int)mp.x - ((int)cross.x / 2) - r + px);
             const double oy = -((int)mp.y - ((int)cross.y / 2) - r + py);
@@ -377,7 +491,6 @@ __global__ __launch_bounds__(NT) void fft_kernel(fft_args<T, IN, SUM> a) {
 }
 
 // --------------------------------------------------------------- launchers
-constexpr int kFftNT = 256;
 constexpr size_t kMaxLds = 160 * 1024;
 
 int fft_max_side(bool is_double) {
@@ -387,11 +500,13 @@ int fft_max_side(bool is_double) {
 
 template <class T, class IN, class SUM, int M, bool FP, bool PIPE>
 void launch_fft_m(const fft_args<T, IN, SUM>& a, uint32_t grid, hipStream_t st) {
-    using G = fft_geom<T, M>;
-    static_assert(G::lds_bytes <= kMaxLds, "FFT working set exceeds LDS");
-    auto kern = fft_kernel<T, IN, SUM, M, kFftNT, FP, PIPE>;
-    EMIDA_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)G::lds_bytes));
-    kern<<<grid, kFftNT, G::lds_bytes, st>>>(a);
+    using P = fft_plan<T, M>;
+    static_assert(P::lds_bytes + 2048 <= kMaxLds, "FFT working set exceeds LDS");
+    auto kern = fft_kernel<T, IN, SUM, M, FP, PIPE>;
+    static const hipError_t attr =
+        hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P::lds_bytes);
+    EMIDA_HIP(attr);
+    kern<<<grid, P::NT, P::lds_bytes, st>>>(a);
     EMIDA_LAUNCHED("fft_kernel");
 }
 
@@ -484,8 +599,7 @@ __global__ void hadamard_kernel(cplx<T>* __restrict__ pics, const cplx<T>* __res
                                 size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const cplx<T> p = pics[i], r = ref[i % ref_len];
-    pics[i] = cplx<T>{p.re * r.re + p.im * r.im, -p.re * r.im + p.im * r.re};
+    pics[i] = hadamard(pics[i], ref[i % ref_len]);
 }
 
 template <class T>

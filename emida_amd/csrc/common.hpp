@@ -121,4 +121,32 @@ __device__ inline void block_argmax(T& v, uint32_t& i, T* sv, uint32_t* si) {
     __syncthreads();
 }
 
+// Workgroup maximum (IEEE maxNum: NaN operands are ignored); every thread receives it.
+template <class T, int NT>
+__device__ inline T block_max(T v, T* sv) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    if (threadIdx.x % kWave == 0) sv[threadIdx.x / kWave] = v;
+    __syncthreads();
+    v = sv[0];
+#pragma unroll
+    for (int k = 1; k < NT / kWave; ++k) v = fmax(v, sv[k]);
+    __syncthreads();
+    return v;
+}
+
+// Workgroup minimum of unsigned values; every thread receives it.
+template <int NT>
+__device__ inline uint32_t block_min_u32(uint32_t v, uint32_t* sv) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off));
+    if (threadIdx.x % kWave == 0) sv[threadIdx.x / kWave] = v;
+    __syncthreads();
+    v = sv[0];
+#pragma unroll
+    for (int k = 1; k < NT / kWave; ++k) v = min(v, sv[k]);
+    __syncthreads();
+    return v;
+}
+
 }  // namespace emida_amd

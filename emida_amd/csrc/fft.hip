@@ -13,7 +13,7 @@
 // one wave, each lane holding R = M/L values: an R-point DFT in registers over
 // n = L*n1 + n2 (lane n2), a twiddle W_M^(n2*k1), one transpose through the group's
 // LDS scratch, and L-point DFTs in registers.  Lane j then holds X[k] for
-// k = (j*Q + i) + R*k2 (Q = R/L, i < Q, k2 < L) - the "frequency layout".  The
+// k = (j + L*i) + R*k2 (i < R/L, k2 < L) - the "frequency layout".  The
 // inverse starts from that layout and ends in natural order, so the product with the
 // reference spectrum (stored in the same layout) needs no reordering.
 //
@@ -23,7 +23,10 @@
 //  A. rows 2p and 2p+1 packed as one complex row: S/2 forward transforms;
 //  B. column kx of the half spectrum, split from the packed rows
 //     (X[k] and conj(X[M-k])); columns 0 and S are real and packed into one
-//     complex column; forward transform, product with conj(reference), inverse;
+//     complex column; forward transform, product with conj(reference), inverse.
+//     Both split steps omit their factor 1/2, so column spectra come out doubled;
+//     the stored reference spectrum is scaled by 1/4 (half its true value), and the
+//     product is exact in scale (powers of two only);
 //  C. rows 2q and 2q+1 of the column result packed as Ga + i Gb (Hermitian
 //     extension, DC and Nyquist imaginary parts dropped as a C2R does), inverse
 //     transform yields both real correlation rows;
@@ -86,11 +89,16 @@ __host__ __device__ constexpr int brev_c(int v, int bits) {
 
 // In-register DFT of N <= 16 points, natural order in and out (radix-2
 // decimation in frequency; the final bit reversal only renames registers).
-template <class T, int N, bool INV>
+template <class T, int N, bool INV, bool HALF_ZERO = false>
 __device__ __forceinline__ void dft_reg(cplx<T> (&x)[N]) {
     static_assert(N >= 1 && N <= 16 && (N & (N - 1)) == 0, "N must be a power of two <= 16");
+    if constexpr (HALF_ZERO && N >= 2) {
+        // first pass with x[N/2..N-1] = 0: x[k] unchanged, x[k + N/2] = x[k] W^k
 #pragma unroll
-    for (int len = N; len >= 2; len >>= 1) {
+        for (int k = 0; k < N / 2; ++k) x[k + N / 2] = twiddle16<T, INV>(x[k], k * (16 / N));
+    }
+#pragma unroll
+    for (int len = HALF_ZERO ? N / 2 : N; len >= 2; len >>= 1) {
         const int half = len / 2;
 #pragma unroll
         for (int s0 = 0; s0 < N; s0 += len) {
@@ -117,11 +125,12 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // Forward transform: lane n2 holds x[n1] = input[L*n1 + n2]; on return lane j holds
-// X[(j*Q + i) + R*k2] in x[i*L + k2].  scr: L*(R+1) elements of this group.
-template <class T, int R, int L, int M>
+// X[(j + L*i) + R*k2] in x[i*L + k2].  scr: L*(R+1) elements of this group.
+// HALF_ZERO: the inputs x[R/2..R-1] are zero padding.
+template <class T, int R, int L, int M, bool HALF_ZERO>
 __device__ __forceinline__ void fft_fwd(cplx<T> (&x)[R], int lane, cplx<T>* scr, const cplx<T>* tw) {
     constexpr int Q = R / L;
-    dft_reg<T, R, false>(x);
+    dft_reg<T, R, false, HALF_ZERO>(x);
 #pragma unroll
     for (int k1 = 1; k1 < R; ++k1) x[k1] = cmul(x[k1], tw[lane * k1]);
 #pragma unroll
@@ -131,7 +140,7 @@ __device__ __forceinline__ void fft_fwd(cplx<T> (&x)[R], int lane, cplx<T>* scr,
 #pragma unroll
     for (int i = 0; i < Q; ++i)
 #pragma unroll
-        for (int m = 0; m < L; ++m) y[i][m] = scr[m * (R + 1) + lane * Q + i];
+        for (int m = 0; m < L; ++m) y[i][m] = scr[m * (R + 1) + lane + L * i];
     wave_sync();
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -152,7 +161,7 @@ __device__ __forceinline__ void fft_inv(cplx<T> (&x)[R], int lane, cplx<T>* scr,
 #pragma unroll
         for (int k2 = 0; k2 < L; ++k2) y[k2] = x[i * L + k2];
         dft_reg<T, L, true>(y);
-        const int k1 = lane * Q + i;
+        const int k1 = lane + L * i;
 #pragma unroll
         for (int n2 = 0; n2 < L; ++n2) scr[n2 * (R + 1) + k1] = n2 == 0 ? y[0] : cmul_conj(y[n2], tw[n2 * k1]);
     }
@@ -177,6 +186,18 @@ template <class T>
 __device__ __forceinline__ T mean_of(T sum, uint32_t area) {
 #pragma clang fp contract(off)
     return sum / (T)area;
+}
+
+// Workgroup -> slice (pattern * nb + roi).  Workgroups are dispatched round robin
+// over the 8 XCDs (b mod 8); XCD x takes a contiguous run of the ROI-major order
+// (roi * npat + pattern), so the patterns of a ROI run back to back on one XCD and
+// that ROI's reference spectrum stays in the XCD's L2.
+__device__ __forceinline__ uint32_t xcd_slice(uint32_t b, uint32_t total, uint32_t nb) {
+    constexpr uint32_t X = 8;
+    const uint32_t x = b % X, l = b / X, per = total / X, rem = total % X;
+    const uint32_t o = x * per + (x < rem ? x : rem) + l;
+    const uint32_t npat = total / nb;
+    return (o % npat) * nb + o / npat;
 }
 
 template <class T, class IN, class SUM>
@@ -219,10 +240,15 @@ struct fft_plan {
     static constexpr int P_OFF = NC * CS - S * S / 2;  // prepared ROI (S*S reals) at the end
     static_assert(P_OFF >= (S / 2) * CS, "prepared ROI overlaps the row stage");
     static constexpr size_t lds_bytes = (size_t)(NC * CS + M) * sizeof(cplx<T>);
+    // waves per SIMD the LDS allows (160 KiB per CU, 4 SIMDs); registers are
+    // budgeted for exactly that many
+    static constexpr int WG_PER_CU = (int)((160 * 1024) / (lds_bytes + 2048));
+    static constexpr int WAVES_PER_SIMD_RAW = WG_PER_CU * (NT / 64) / 4;
+    static constexpr int WAVES_PER_SIMD = WAVES_PER_SIMD_RAW < 1 ? 1 : (WAVES_PER_SIMD_RAW > 4 ? 4 : WAVES_PER_SIMD_RAW);
 };
 
 template <class T, class IN, class SUM, int M, bool FROM_PATTERN, bool PIPE>
-__global__ __launch_bounds__((fft_plan<T, M>::NT)) void fft_kernel(fft_args<T, IN, SUM> a) {
+__global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SIMD)) void fft_kernel(fft_args<T, IN, SUM> a) {
     using P = fft_plan<T, M>;
     using C = cplx<T>;
     constexpr int S = P::S, L = P::L, R = P::R, Q = P::Q, CS = P::CS, NT = P::NT;
@@ -236,8 +262,8 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT)) void fft_kernel(fft_args<T, I
     __shared__ T nbv[81];
     __shared__ double cf[6];
 
-    const uint32_t sn = blockIdx.x;
-    if (sn >= a.total) return;
+    if (blockIdx.x >= a.total) return;
+    const uint32_t sn = xcd_slice(blockIdx.x, a.total, a.nb);
     const uint32_t bn = sn % a.nb;
     const int tid = threadIdx.x;
     const int lane = tid % L, grp = tid / L;
@@ -301,11 +327,11 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT)) void fft_kernel(fft_args<T, I
             }
         }
         C* slot = B + p * CS;
-        fft_fwd<T, R, L, M>(x, lane, slot, tw);
+        fft_fwd<T, R, L, M, true>(x, lane, slot, tw);
 #pragma unroll
         for (int i = 0; i < Q; ++i)
 #pragma unroll
-            for (int k2 = 0; k2 < L; ++k2) slot[(lane * Q + i) + R * k2] = x[i * L + k2];
+            for (int k2 = 0; k2 < L; ++k2) slot[(lane + L * i) + R * k2] = x[i * L + k2];
     }
     __syncthreads();
 
@@ -323,8 +349,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT)) void fft_kernel(fft_args<T, I
                     x[n1] = (y & 1) ? C{z0.im, zs.im} : C{z0.re, zs.re};
                 } else {
                     const C zk = row[c], zm = row[M - c];
-                    x[n1] = (y & 1) ? C{(zk.im + zm.im) * T(0.5), (zm.re - zk.re) * T(0.5)}
-                                    : C{(zk.re + zm.re) * T(0.5), (zk.im - zm.im) * T(0.5)};
+                    x[n1] = (y & 1) ? C{zk.im + zm.im, zm.re - zk.re} : C{zk.re + zm.re, zk.im - zm.im};
                 }
             } else {
                 x[n1] = C{T(0), T(0)};
@@ -334,24 +359,24 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT)) void fft_kernel(fft_args<T, I
     __syncthreads();  // row spectra consumed; slots become column scratch
     if (c < S) {
         C* slot = B + c * CS;
-        fft_fwd<T, R, L, M>(x, lane, slot, tw);
+        fft_fwd<T, R, L, M, true>(x, lane, slot, tw);
         if (c == 0) {
             // split the packed real columns: P0[k] = (Z[k] + conj(Z[-k]))/2,
             // PS[k] = (Z[k] - conj(Z[-k]))/(2i)
 #pragma unroll
             for (int i = 0; i < Q; ++i)
 #pragma unroll
-                for (int k2 = 0; k2 < L; ++k2) slot[(lane * Q + i) + R * k2] = x[i * L + k2];
+                for (int k2 = 0; k2 < L; ++k2) slot[(lane + L * i) + R * k2] = x[i * L + k2];
             wave_sync();
             C p0[R], ps[R];
 #pragma unroll
             for (int i = 0; i < Q; ++i)
 #pragma unroll
                 for (int k2 = 0; k2 < L; ++k2) {
-                    const int k = (lane * Q + i) + R * k2;
+                    const int k = (lane + L * i) + R * k2;
                     const C z = x[i * L + k2], zm = slot[(M - k) & (M - 1)];
-                    p0[i * L + k2] = C{(z.re + zm.re) * T(0.5), (z.im - zm.im) * T(0.5)};
-                    ps[i * L + k2] = C{(z.im + zm.im) * T(0.5), (zm.re - z.re) * T(0.5)};
+                    p0[i * L + k2] = C{z.re + zm.re, z.im - zm.im};
+                    ps[i * L + k2] = C{z.im + zm.im, zm.re - z.re};
                 }
             wave_sync();
             if constexpr (PIPE) {
@@ -367,8 +392,8 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT)) void fft_kernel(fft_args<T, I
                 C* os = a.spectra_out + ((size_t)bn * P::NC + S) * M + lane * R;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    o0[r] = p0[r];
-                    os[r] = ps[r];
+                    o0[r] = C{p0[r].re * T(0.25), p0[r].im * T(0.25)};
+                    os[r] = C{ps[r].re * T(0.25), ps[r].im * T(0.25)};
                 }
             }
         } else {
@@ -379,7 +404,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT)) void fft_kernel(fft_args<T, I
             } else {
                 C* oc = a.spectra_out + ((size_t)bn * P::NC + c) * M + lane * R;
 #pragma unroll
-                for (int r = 0; r < R; ++r) oc[r] = x[r];
+                for (int r = 0; r < R; ++r) oc[r] = C{x[r].re * T(0.25), x[r].im * T(0.25)};
             }
         }
         if constexpr (PIPE) {
@@ -410,7 +435,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT)) void fft_kernel(fft_args<T, I
             for (int i = 0; i < Q; ++i)
 #pragma unroll
                 for (int k2 = 0; k2 < L; ++k2) {
-                    const int k = (lane * Q + i) + R * k2;
+                    const int k = (lane + L * i) + R * k2;
                     const bool mirror = k > S;
                     const C* col = B + (mirror ? M - k : k) * CS;
                     C ga = col[2 * q], gb = col[2 * q + 1];
@@ -426,8 +451,9 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT)) void fft_kernel(fft_args<T, I
                 }
         }
         __syncthreads();  // column results consumed; slots become row scratch
-        T best = -(T)__builtin_huge_val();
-        uint32_t bi = 0xffffffffu;
+        // arg-max in two steps: the largest value (v_max, NaN ignored), then the
+        // lowest index holding it, searched only by lanes that hold it
+        T vmax = -(T)__builtin_huge_val();
         if (q < S) {
             C* slot = B + q * CS;
             fft_inv<T, R, L, M>(x, lane, slot, tw);
@@ -436,16 +462,25 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT)) void fft_kernel(fft_args<T, I
             for (int n1 = 0; n1 < R; ++n1) {
                 const int xx = L * n1 + lane;
                 slot[xx] = x[n1];
-                const uint32_t i0 = (uint32_t)(2 * q) * M + xx, i1 = i0 + M;
-                argmax_combine(best, bi, x[n1].re, i0);
-                argmax_combine(best, bi, x[n1].im, i1);
+                vmax = fmax(vmax, fmax(x[n1].re, x[n1].im));
                 if (cout) {
+                    const uint32_t i0 = (uint32_t)(2 * q) * M + xx;
                     cout[i0] = x[n1].re;
-                    cout[i1] = x[n1].im;
+                    cout[i0 + M] = x[n1].im;
                 }
             }
         }
-        block_argmax<T, NT>(best, bi, sv, si);
+        const T best = block_max<T, NT>(vmax, sv);
+        uint32_t bi = 0xffffffffu;
+        if (q < S && vmax == best) {
+#pragma unroll
+            for (int n1 = 0; n1 < R; ++n1) {
+                const uint32_t i0 = (uint32_t)(2 * q) * M + L * n1 + lane;
+                if (x[n1].re == best) bi = min(bi, i0);
+                if (x[n1].im == best) bi = min(bi, i0 + M);
+            }
+        }
+        bi = block_min_u32<NT>(bi, si);
 
         // ---- D. gather (FFT position policy, clamp to [0, 2s-1]) and fit
         const int fit = a.fit, kk = fit * fit, r = (fit - 1) / 2;

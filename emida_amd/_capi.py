@@ -10,7 +10,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libemida_amd.so")
 
 F32, F64, I32, U16 = 0, 1, 2, 3
-CROSS_POLICY_BRUTE, CROSS_POLICY_FFT = 0, 1
+CROSS_POLICY_BRUTE, CROSS_POLICY_FFT, CROSS_POLICY_BRUTE_MFMA = 0, 1, 2
+MFMA_BF16, MFMA_BF16_SPLIT, MFMA_F16, MFMA_F16_SPLIT = 0, 1, 2, 3
 OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED, ERR_NOMEM = range(6)
 
 _TYPE_OF = {np.dtype(np.float32): F32, np.dtype(np.float64): F64, np.dtype(np.int32): I32,
@@ -74,6 +75,7 @@ def lib():
             "emida_offset_finalize": ([vp, vp, vp, dp, dp], i32),
             "emida_offset_get_offset": ([vp, vp, dp, dp], i32),
             "emida_offset_process_device": ([vp, vp, u32, vp, vp, vp, vp, vp], i32),
+            "emida_offset_set_mfma_mode": ([vp, i32], i32),
             "emida_offset_set_timing": ([vp, i32], i32),
             "emida_offset_get_timing": ([vp, ctypes.POINTER(ctypes.c_char_p), dp, ctypes.POINTER(u64)], i32),
             "emida_host_alloc": ([ctypes.POINTER(vp), sz], i32),
@@ -90,6 +92,7 @@ def lib():
             "emida_run_prepare_pics": ([i32, i32, i32, vp, vp, vp, vp, vp, vp, S, S, S, u32, u32, vp], i32),
             "emida_run_cross_corr": ([i32, vp, vp, vp, S, S, u32, u32, vp], i32),
             "emida_run_cross_corr_fft": ([i32, vp, vp, vp, S, u32, u32, vp], i32),
+            "emida_run_cross_corr_mfma": ([i32, vp, vp, vp, S, S, u32, u32, vp], i32),
             "emida_run_hadamard": ([i32, vp, vp, S, u32, u32, vp], i32),
             "emida_run_maxarg_reduce": ([i32, vp, vp, S, u32, vp], i32),
             "emida_run_extract_neighbors": ([i32, i32, vp, sz, vp, vp, i32, S, u32, vp], i32),

@@ -99,6 +99,20 @@ void with_IN(emida_type IN, F&& f) {
     else throw arg_error("IN must be U16, F32 or F64");
 }
 
+template <class X>
+struct tag {
+    using type = X;
+};
+
+// Half type of a matrix-core mode.
+template <class F>
+void with_HT(emida_mfma_mode mode, F&& f) {
+    if (mode == EMIDA_MFMA_BF16 || mode == EMIDA_MFMA_BF16_SPLIT) f(tag<__bf16>{});
+    else if (mode == EMIDA_MFMA_F16 || mode == EMIDA_MFMA_F16_SPLIT) f(tag<_Float16>{});
+    else throw arg_error("unknown matrix-core mode");
+}
+bool mfma_split(emida_mfma_mode mode) { return mode == EMIDA_MFMA_BF16_SPLIT || mode == EMIDA_MFMA_F16_SPLIT; }
+
 void* dmalloc(size_t bytes) {
     void* p = nullptr;
     if (bytes == 0) bytes = 16;
@@ -136,6 +150,10 @@ struct emida_offset_s {
     void* d_slices = nullptr;  // brute: prepared pattern slices
     void* d_corr = nullptr;    // brute: correlation buffers + zero tail
     size_t corr_elems = 0;
+    emida_mfma_mode mfma_mode = EMIDA_MFMA_BF16_SPLIT;
+    void* d_ref_lo = nullptr;  // matrix-core policy: lo halves of the reference slices
+    void* d_pic_hi = nullptr;  // matrix-core policy: prepared pattern slices, hi / lo halves
+    void* d_pic_lo = nullptr;
 
     struct outset {
         void* maxes = nullptr;
@@ -184,7 +202,7 @@ struct emida_offset_s {
         for (void* p : d_in) (void)hipFree(p);
         for (auto e : ev_up) (void)hipEventDestroy(e);
         for (auto e : ev_done) (void)hipEventDestroy(e);
-        void* ptrs[] = {d_begins, d_hx, d_hy, d_sums, d_ref, d_slices, d_corr};
+        void* ptrs[] = {d_begins, d_hx, d_hy, d_sums, d_ref, d_slices, d_corr, d_ref_lo, d_pic_hi, d_pic_lo};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         for (auto& o : out) {
@@ -227,6 +245,14 @@ struct emida_offset_s {
                     launch_sum<IT, ST>(pic, sums, d_begins, src, slice, nb, nb, compute);
                     launch_prepare<IT, TT, ST>(pic, (TT*)d_ref, (const TT*)d_hx, (const TT*)d_hy, sums, d_begins,
                                                src, slice, slice, nb, nb, compute);
+                } else if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA) {
+                    launch_sum<IT, ST>(pic, sums, d_begins, src, slice, nb, nb, compute);
+                    with_HT(mfma_mode, [&](auto ht) {
+                        using HT = typename decltype(ht)::type;
+                        launch_prepare_split<IT, ST, HT>(pic, (HT*)d_ref, (HT*)d_ref_lo, (const float*)d_hx,
+                                                         (const float*)d_hy, sums, d_begins, src, slice, nb, nb,
+                                                         compute);
+                    });
                 } else {
                     if (!std::is_same<IT, uint16_t>::value)
                         launch_sum<IT, ST>(pic, sums, d_begins, src, slice, nb, nb, compute);
@@ -272,6 +298,17 @@ struct emida_offset_s {
             corr_elems = (size_t)total * cross.area() + cross.x + 1;
             d_corr = alloc(corr_elems * t_bytes());
             EMIDA_HIP(hipMemset(d_corr, 0, corr_elems * t_bytes()));
+        } else if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA) {
+            const size_t half = 2;  // bf16 / fp16
+            d_ref = alloc((size_t)nb * slice.area() * half);
+            d_pic_hi = alloc((size_t)total * slice.area() * half);
+            if (mfma_split(mfma_mode)) {
+                d_ref_lo = alloc((size_t)nb * slice.area() * half);
+                d_pic_lo = alloc((size_t)total * slice.area() * half);
+            }
+            corr_elems = (size_t)total * cross.area() + cross.x + 1;
+            d_corr = alloc(corr_elems * sizeof(float));
+            EMIDA_HIP(hipMemset(d_corr, 0, corr_elems * sizeof(float)));
         } else {
             const size_t spec = (size_t)(side + 1) * (2 * side) * 2 * t_bytes();
             d_ref = alloc(spec * nb);
@@ -311,6 +348,20 @@ struct emida_offset_s {
                     if (timing) record_stop(st, t0);
                     launch_peak<TT>((const TT*)d_corr, (size_t)tn * cross.area(), cross.area(), cross, POLICY_BRUTE,
                                     fit, (data_index<TT>*)maxes, (TT*)neigh, offsets, coefs, tn, st);
+                } else if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA) {
+                    launch_sum<IT, ST>(pics, sums, d_begins, src, slice, nb, tn, st);
+                    with_HT(mfma_mode, [&](auto ht) {
+                        using HT = typename decltype(ht)::type;
+                        launch_prepare_split<IT, ST, HT>(pics, (HT*)d_pic_hi, (HT*)d_pic_lo, (const float*)d_hx,
+                                                         (const float*)d_hy, sums, d_begins, src, slice, nb, tn, st);
+                        if (timing) t0 = record_start(st);
+                        launch_mfma_corr<HT>((const HT*)d_pic_hi, (const HT*)d_pic_lo, (const HT*)d_ref,
+                                             (const HT*)d_ref_lo, (float*)d_corr, slice, cross, nb, n_pat, st);
+                        if (timing) record_stop(st, t0);
+                    });
+                    launch_peak<float>((const float*)d_corr, (size_t)tn * cross.area(), cross.area(), cross,
+                                       POLICY_BRUTE, fit, (data_index<float>*)maxes, (float*)neigh, offsets, coefs,
+                                       tn, st);
                 } else {
                     if (!std::is_same<IT, uint16_t>::value)
                         launch_sum<IT, ST>(pics, sums, d_begins, src, slice, nb, tn, st);
@@ -418,7 +469,7 @@ struct emida_offset_s {
         try {
             EMIDA_HIP(hipMemcpyAsync(dm, maxes_i, total * ti_bytes(), hipMemcpyHostToDevice, compute));
             EMIDA_HIP(hipMemcpyAsync(dn, neighbors, total * nk * t_bytes(), hipMemcpyHostToDevice, compute));
-            const int pol = policy == EMIDA_CROSS_POLICY_BRUTE ? POLICY_BRUTE : POLICY_FFT;
+            const int pol = policy == EMIDA_CROSS_POLICY_FFT ? POLICY_FFT : POLICY_BRUTE;
             with_T(T, [&](auto t) {
                 using TT = decltype(t);
                 launch_subpixel<TT>((const TT*)dn, fit, total, (const data_index<TT>*)dm, cross, pol, doff, dco,
@@ -456,7 +507,11 @@ int emida_offset_create(emida_offset** out, emida_type T, emida_type IN, emida_s
         *out = nullptr;
         if (T != EMIDA_F32 && T != EMIDA_F64) throw arg_error("T must be F32 or F64");
         if (IN != EMIDA_U16 && IN != EMIDA_F32 && IN != EMIDA_F64) throw arg_error("IN must be U16, F32 or F64");
-        if (policy != EMIDA_CROSS_POLICY_BRUTE && policy != EMIDA_CROSS_POLICY_FFT) throw arg_error("unknown policy");
+        if (policy != EMIDA_CROSS_POLICY_BRUTE && policy != EMIDA_CROSS_POLICY_FFT &&
+            policy != EMIDA_CROSS_POLICY_BRUTE_MFMA)
+            throw arg_error("unknown policy");
+        if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA && T != EMIDA_F32)
+            throw unsupported_error("matrix-core brute policy: results are float (T must be F32)");
         if (fitsize != 3 && fitsize != 5 && fitsize != 7 && fitsize != 9)
             throw arg_error("fitsize must be 3, 5, 7 or 9");
         if (!begins || n_begins == 0) throw arg_error("at least one ROI is required");
@@ -481,7 +536,8 @@ int emida_offset_create(emida_offset** out, emida_type T, emida_type IN, emida_s
         h->begins.resize(n_begins);
         for (uint32_t i = 0; i < n_begins; ++i) h->begins[i] = S2(begins[i]);
         h->sum_bytes = IN == EMIDA_U16 ? 4 : type_size(IN);
-        if (policy == EMIDA_CROSS_POLICY_BRUTE) {
+        if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA) check_mfma_shape(S2(slice_size), S2(cross_size));
+        if (policy != EMIDA_CROSS_POLICY_FFT) {
             if (cross_size.x == 0 || cross_size.y == 0) throw arg_error("cross size must be positive");
             if (cross_size.x > 2 * slice_size.x - 1 || cross_size.y > 2 * slice_size.y - 1)
                 throw arg_error("cross size must be at most 2*slice-1 per axis");
@@ -594,6 +650,15 @@ int emida_offset_process_device(emida_offset* h, const void* d_pics, uint32_t n_
     });
 }
 
+int emida_offset_set_mfma_mode(emida_offset* h, emida_mfma_mode mode) {
+    return guarded([&] {
+        if (!h) throw arg_error("null object");
+        if (h->allocated) throw state_error("set_mfma_mode must precede allocate_memory");
+        with_HT(mode, [](auto) {});
+        h->mfma_mode = mode;
+    });
+}
+
 int emida_offset_set_timing(emida_offset* h, int enable) {
     return guarded([&] {
         if (!h) throw arg_error("null object");
@@ -614,7 +679,10 @@ int emida_offset_get_timing(emida_offset* h, const char** kernel_name, double* t
             (void)hipEventDestroy(p.second);
         }
         h->tev.clear();
-        if (kernel_name) *kernel_name = h->policy == EMIDA_CROSS_POLICY_FFT ? "fft_kernel" : "cross_corr_kernel";
+        if (kernel_name)
+            *kernel_name = h->policy == EMIDA_CROSS_POLICY_FFT
+                               ? "fft_kernel"
+                               : (h->policy == EMIDA_CROSS_POLICY_BRUTE_MFMA ? "mfma_corr_kernel" : "cross_corr_kernel");
         if (total_ms) *total_ms = h->t_ms;
         if (launches) *launches = h->t_launches;
     });
@@ -737,6 +805,34 @@ int emida_run_cross_corr_fft(emida_type T, const void* pics, const void* ref, vo
             using TT = decltype(t);
             launch_fft_corr_prepared<TT>((const TT*)pics, (const TT*)ref, (TT*)res, size.x, ref_slices, batch_size,
                                          (hipStream_t)stream);
+        });
+    });
+}
+
+int emida_run_cross_corr_mfma(emida_mfma_mode mode, const float* pics, const float* ref, float* res,
+                              emida_size2 size, emida_size2 res_size, uint32_t ref_slices, uint32_t batch_size,
+                              void* stream) {
+    return guarded([&] {
+        if (!pics || !ref || !res) throw arg_error("null argument");
+        check_mfma_shape(S2(size), S2(res_size));
+        hipStream_t st = (hipStream_t)stream;
+        const size_t area = (size_t)size.x * size.y, np = area * ref_slices * batch_size, nr = area * ref_slices;
+        if (np == 0) return;
+        const bool split = mfma_split(mode);
+        with_HT(mode, [&](auto ht) {
+            using HT = typename decltype(ht)::type;
+            HT *ph = nullptr, *pl = nullptr, *rh = nullptr, *rl = nullptr;
+            EMIDA_HIP(hipMallocAsync((void**)&ph, np * sizeof(HT), st));
+            EMIDA_HIP(hipMallocAsync((void**)&rh, nr * sizeof(HT), st));
+            if (split) {
+                EMIDA_HIP(hipMallocAsync((void**)&pl, np * sizeof(HT), st));
+                EMIDA_HIP(hipMallocAsync((void**)&rl, nr * sizeof(HT), st));
+            }
+            launch_split<HT>(pics, ph, pl, np, st);
+            launch_split<HT>(ref, rh, rl, nr, st);
+            launch_mfma_corr<HT>(ph, pl, rh, rl, res, S2(size), S2(res_size), ref_slices, batch_size, st);
+            for (HT* p : {ph, pl, rh, rl})
+                if (p) EMIDA_HIP(hipFreeAsync(p, st));
         });
     });
 }

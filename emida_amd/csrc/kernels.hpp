@@ -65,4 +65,17 @@ template <class T>
 void launch_hadamard(cplx<T>* pics, const cplx<T>* ref, size2 one, uint32_t ref_slices, uint32_t batch,
                      hipStream_t st);
 
+// ---- mfma.hip (matrix-core brute correlation, float results)
+// Prepared slices as half-type hi (+ lo when lo != nullptr) planes [slice][y][x];
+// corr: [pattern * nb + roi][cross.y][cross.x].
+template <class HT>
+void launch_mfma_corr(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
+                      uint32_t nb, uint32_t npat, hipStream_t st);
+void check_mfma_shape(size2 s, size2 cross);
+template <class IN, class S, class HT>
+void launch_prepare_split(const IN* pics, HT* hi, HT* lo, const float* hx, const float* hy, const S* sums,
+                          const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st);
+template <class HT>
+void launch_split(const float* v, HT* hi, HT* lo, size_t n, hipStream_t st);
+
 }  // namespace emida_amd

@@ -1,0 +1,295 @@
+// Brute-force correlation as an implicit GEMM on CDNA4 matrix cores ("precision
+// form"): BF16 or FP16 operands, FP32 accumulation, optionally split into hi + lo
+// halves with three products (hi*hi + hi*lo + lo*hi) to win back precision.
+//
+// Reference semantics: src/cross_corr.cu:11-66, res[sy][sx] = sum over the overlap
+// of pic[y + sy][x + sx] * ref[y][x].  For one ROI and a batch of patterns p this is
+//   C[p][sy][sx] = sum_y sum_k P_p[y + sy][k] * B_y[k][sx],   B_y[k][sx] = R[y][k - sx]
+// i.e. a GEMM with M = patterns, N = x-shifts and K = pixel columns, summed over the
+// reference rows y, where B_y is the Toeplitz matrix of reference row y (zero outside
+// the row).  All-zero Toeplitz tiles are skipped.
+//
+// Work decomposition: one workgroup = 32 patterns x every x-shift (4 waves x two
+// 32-wide tiles) x TS consecutive y-shifts; blockIdx.x = ROI + nb * pattern tile,
+// blockIdx.y = y-shift tile.  It walks the reference rows y; the
+// pattern rows Y = y + sy of its y-shifts live in an LDS ring (one new row per step),
+// and reference row y is staged as 8 shifted copies so that any 8-element window -
+// one lane's B fragment - is a single aligned 16-byte LDS read.  B fragments are
+// loaded once per (y, K tile) and reused for all TS y-shifts.
+//
+// Matrix-core operand layout (v_mfma_f32_32x32x16_{bf16,f16}): lane l holds
+// A[l % 32][8 (l / 32) + t] and B[8 (l / 32) + t][l % 32] for t < 8; the 16
+// accumulators are D[8 (j / 4) + 4 (l / 32) + j % 4][l % 32], j < 16.
+#include <type_traits>
+
+#include "kernels.hpp"
+
+namespace emida_amd {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template <class HT>
+struct mfma_vec;
+template <>
+struct mfma_vec<__bf16> {
+    using type = bf16x8;
+    __device__ static f32x16 mma(bf16x8 a, bf16x8 b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct mfma_vec<_Float16> {
+    using type = f16x8;
+    __device__ static f32x16 mma(f16x8 a, f16x8 b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+// Prepared value in float, the reference's order (src/prepare_pics.cu:36-41), split
+// into hi = round(v) and lo = round(v - hi) in the half type.
+template <class IN, class S, class HT>
+__global__ __launch_bounds__(256) void prepare_split_kernel(const IN* __restrict__ pics, HT* __restrict__ hi,
+                                                            HT* __restrict__ lo, const float* __restrict__ hx,
+                                                            const float* __restrict__ hy, const S* __restrict__ sums,
+                                                            const size2* __restrict__ begins, size2 src, size2 slice,
+                                                            uint32_t nb, uint32_t total) {
+#pragma clang fp contract(off)
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t area = slice.area();
+    const size_t sn = tid / area;
+    if (sn >= total) return;
+    const uint32_t st = (uint32_t)(tid % area);
+    const uint32_t x = st % slice.x, y = st / slice.x;
+    const size2 b0 = begins[sn % nb];
+    float v = (float)pics[(sn / nb) * src.area() + (size_t)(b0.y + y) * src.x + b0.x + x];
+    v -= (float)sums[sn] / (float)area;
+    v = v * hx[x] * hy[y];
+    const HT h = (HT)v;
+    hi[tid] = h;
+    if (lo) lo[tid] = (HT)(v - (float)h);
+}
+
+constexpr int kMfmaWaves = 4;
+constexpr int kMfmaTilesPerWave = 2;
+constexpr int kMfmaSX = kMfmaWaves * kMfmaTilesPerWave * 32;  // x-shifts per workgroup (256)
+constexpr int kMfmaM = 32;                                    // patterns per workgroup
+
+template <class HT, bool SPLIT, int TS>
+struct mfma_plan {
+    static constexpr int KMAX = 128;          // slice width limit (K)
+    static constexpr int AROW = KMAX + 8;     // LDS row stride of a pattern row (elements)
+    static constexpr int RP = KMAX + kMfmaSX + 8;  // length of one shifted reference copy
+    static constexpr int NH = SPLIT ? 2 : 1;  // hi (+ lo) planes
+    static constexpr size_t lds_a = (size_t)NH * TS * kMfmaM * AROW * sizeof(HT);
+    static constexpr size_t lds_r = (size_t)NH * 8 * RP * sizeof(HT);
+    static constexpr size_t lds_bytes = lds_a + lds_r;
+};
+
+// hi/lo: prepared pattern slices [slice][y][x] (slice = pattern * nb + roi);
+// rhi/rlo: prepared reference slices [roi][y][x]; corr: [slice][cross.y][cross.x].
+template <class HT, bool SPLIT, int TS>
+__global__ __launch_bounds__(kMfmaWaves * 64) void mfma_corr_kernel(const HT* __restrict__ phi,
+                                                                    const HT* __restrict__ plo,
+                                                                    const HT* __restrict__ rhi,
+                                                                    const HT* __restrict__ rlo,
+                                                                    float* __restrict__ corr, size2 s, size2 cross,
+                                                                    uint32_t nb, uint32_t npat) {
+    using P = mfma_plan<HT, SPLIT, TS>;
+    using V = typename mfma_vec<HT>::type;
+    constexpr int NT = kMfmaWaves * 64;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    HT* A = reinterpret_cast<HT*>(lds_raw);                 // [NH][TS][32][AROW]
+    HT* Rs = reinterpret_cast<HT*>(lds_raw + P::lds_a);     // [NH][8][RP]
+
+    const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+    const uint32_t roi = blockIdx.x % nb, mt = blockIdx.x / nb;
+    const int rx = (int)(cross.x - 1) / 2, ry = (int)(cross.y - 1) / 2;
+    const int sy0 = (int)blockIdx.y * TS - ry;
+    const int sxg0 = -rx;       // first x-shift of the workgroup
+    const int off = 256 - rx;   // shifted-copy offset, see below
+    const int S = (int)s.x, SY = (int)s.y;
+    const int K16 = (S + 15) / 16 * 16;
+    const uint32_t p0 = mt * kMfmaM;
+    const size_t slice_elems = (size_t)s.x * s.y;
+    const HT* refh = rhi + (size_t)roi * slice_elems;
+    const HT* refl = SPLIT ? rlo + (size_t)roi * slice_elems : nullptr;
+
+    f32x16 acc[TS][kMfmaTilesPerWave];
+#pragma unroll
+    for (int t = 0; t < TS; ++t)
+#pragma unroll
+        for (int u = 0; u < kMfmaTilesPerWave; ++u) acc[t][u] = f32x16{0};
+
+    // stage pattern row Y of the 32 patterns into ring slot Y mod TS (zeros past K)
+    auto stage_row = [&](int Y) {
+        const int slot = ((Y % TS) + TS) % TS;
+        constexpr int CH = P::AROW / 8;  // 8-element chunks per LDS row
+        for (int e = tid; e < kMfmaM * CH; e += NT) {
+            const int m = e / CH, c = e % CH;
+            V vh{}, vl{};
+            const uint32_t p = p0 + m;
+            if (p < npat && Y >= 0 && Y < SY && 8 * c < S) {
+                const size_t g = ((size_t)p * nb + roi) * slice_elems + (size_t)Y * S + 8 * c;
+                vh = *reinterpret_cast<const V*>(phi + g);
+                if constexpr (SPLIT) vl = *reinterpret_cast<const V*>(plo + g);
+            }
+            *reinterpret_cast<V*>(A + ((size_t)slot * kMfmaM + m) * P::AROW + 8 * c) = vh;
+            if constexpr (SPLIT)
+                *reinterpret_cast<V*>(A + ((size_t)(TS + slot) * kMfmaM + m) * P::AROW + 8 * c) = vl;
+        }
+    };
+
+    // y range: some y-shift of the tile overlaps reference row y
+    const int y_lo = max(0, -(sy0 + TS - 1)), y_hi = min(SY, SY - sy0);
+    for (int Y = y_lo + sy0; Y < y_lo + sy0 + TS - 1; ++Y) stage_row(Y);
+    for (int y = y_lo; y < y_hi; ++y) {
+        __syncthreads();  // previous step's reads of the ring slot and reference copies done
+        stage_row(y + sy0 + TS - 1);
+        // reference row y as 8 shifted copies: Rs[rho][i] = R[y][i + rho - off] (0 outside
+        // the row), so the window R[y][q .. q+7] is Rs[rho][jp - rho ..] with jp = q + off,
+        // rho = jp & 7.  q spans [rx - 255, K16 - 8 + rx], hence jp in [1, K16 + 248].
+        for (int e = tid; e < 8 * P::RP; e += NT) {
+            const int rho = e / P::RP, i = e % P::RP;
+            const int col = i + rho - off;
+            HT vh = HT(0.0f), vl = HT(0.0f);
+            if (col >= 0 && col < S) {
+                vh = refh[(size_t)y * S + col];
+                if constexpr (SPLIT) vl = refl[(size_t)y * S + col];
+            }
+            Rs[(size_t)rho * P::RP + i] = vh;
+            if constexpr (SPLIT) Rs[(size_t)(8 + rho) * P::RP + i] = vl;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kMfmaTilesPerWave; ++u) {
+            const int sx0 = sxg0 + (wave * kMfmaTilesPerWave + u) * 32;
+            if (sx0 + rx >= (int)cross.x) continue;
+            for (int k0 = 0; k0 < K16; k0 += 16) {
+                // skip all-zero Toeplitz tiles: need some 0 <= k - sx < S with
+                // k in [k0, k0 + 16) and sx in [sx0, sx0 + 32)
+                if (k0 + 15 < sx0 || k0 - 31 - sx0 > S - 1) continue;
+                const int q = k0 + 8 * (lane / 32) - sx0 - (lane % 32);
+                const int jp = q + off;
+                const int rho = jp & 7;
+                const V bh = *reinterpret_cast<const V*>(Rs + (size_t)rho * P::RP + (jp - rho));
+                V bl{};
+                if constexpr (SPLIT) bl = *reinterpret_cast<const V*>(Rs + (size_t)(8 + rho) * P::RP + (jp - rho));
+#pragma unroll
+                for (int t = 0; t < TS; ++t) {
+                    const int Y = y + sy0 + t;
+                    if (Y < 0 || Y >= SY) continue;
+                    const int slot = Y % TS;
+                    const HT* arow = A + ((size_t)slot * kMfmaM + (lane % 32)) * P::AROW + k0 + 8 * (lane / 32);
+                    const V ah = *reinterpret_cast<const V*>(arow);
+                    acc[t][u] = mfma_vec<HT>::mma(ah, bh, acc[t][u]);
+                    if constexpr (SPLIT) {
+                        const V al = *reinterpret_cast<const V*>(arow + (size_t)TS * kMfmaM * P::AROW);
+                        acc[t][u] = mfma_vec<HT>::mma(ah, bl, acc[t][u]);
+                        acc[t][u] = mfma_vec<HT>::mma(al, bh, acc[t][u]);
+                    }
+                }
+            }
+        }
+    }
+    // write the accumulators: D[8 (j/4) + 4 (l/32) + j%4][l%32]
+#pragma unroll
+    for (int t = 0; t < TS; ++t) {
+        const int sy = sy0 + t;
+        if (sy + ry >= (int)cross.y) continue;
+#pragma unroll
+        for (int u = 0; u < kMfmaTilesPerWave; ++u) {
+            const int sx = sxg0 + (wave * kMfmaTilesPerWave + u) * 32 + (lane % 32);
+            if (sx + rx >= (int)cross.x) continue;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const uint32_t m = 8 * (j / 4) + 4 * (lane / 32) + (j % 4);
+                const uint32_t p = p0 + m;
+                if (p >= npat) continue;
+                corr[(((size_t)p * nb + roi) * cross.y + (sy + ry)) * cross.x + (sx + rx)] = acc[t][u][j];
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------- launchers
+constexpr int kMfmaTS = 2;
+
+template <class HT, bool SPLIT>
+void launch_mfma_corr_t(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
+                        uint32_t nb, uint32_t npat, hipStream_t st) {
+    using P = mfma_plan<HT, SPLIT, kMfmaTS>;
+    auto kern = mfma_corr_kernel<HT, SPLIT, kMfmaTS>;
+    static const hipError_t attr =
+        hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P::lds_bytes);
+    EMIDA_HIP(attr);
+    const uint64_t slabs = (uint64_t)nb * div_up(npat, (uint32_t)kMfmaM);
+    if (slabs > (1u << 22)) throw unsupported_error("matrix-core brute policy: too many ROI x pattern tiles");
+    dim3 grid((uint32_t)slabs, div_up(cross.y, (uint32_t)kMfmaTS));
+    kern<<<grid, kMfmaWaves * 64, P::lds_bytes, st>>>(phi, plo, rhi, rlo, corr, s, cross, nb, npat);
+    EMIDA_LAUNCHED("mfma_corr_kernel");
+}
+
+void check_mfma_shape(size2 s, size2 cross) {
+    if (s.x % 8 != 0 || s.x < 8 || s.x > 128 || s.y < 1)
+        throw unsupported_error("matrix-core brute policy: slice width must be a multiple of 8 in [8, 128]");
+    if (cross.x == 0 || cross.y == 0 || cross.x > 2 * s.x - 1 || cross.y > 2 * s.y - 1)
+        throw arg_error("matrix-core brute policy: cross size must be in [1, 2*slice-1]");
+}
+
+template <class HT>
+void launch_mfma_corr(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
+                      uint32_t nb, uint32_t npat, hipStream_t st) {
+    check_mfma_shape(s, cross);
+    if (nb == 0 || npat == 0) return;
+    if (plo)
+        launch_mfma_corr_t<HT, true>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st);
+    else
+        launch_mfma_corr_t<HT, false>(phi, nullptr, rhi, nullptr, corr, s, cross, nb, npat, st);
+}
+
+template <class IN, class S, class HT>
+void launch_prepare_split(const IN* pics, HT* hi, HT* lo, const float* hx, const float* hy, const S* sums,
+                          const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st) {
+    const size_t n = (size_t)slice.area() * total;
+    if (n == 0) return;
+    prepare_split_kernel<IN, S, HT><<<(unsigned)div_up(n, (size_t)256), 256, 0, st>>>(pics, hi, lo, hx, hy, sums,
+                                                                                     begins, src, slice, nb, total);
+    EMIDA_LAUNCHED("prepare_split_kernel");
+}
+
+// Already prepared float slices -> hi (+ lo) halves.
+template <class HT>
+__global__ void split_kernel(const float* __restrict__ v, HT* __restrict__ hi, HT* __restrict__ lo, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const HT h = (HT)v[i];
+    hi[i] = h;
+    if (lo) lo[i] = (HT)(v[i] - (float)h);
+}
+
+template <class HT>
+void launch_split(const float* v, HT* hi, HT* lo, size_t n, hipStream_t st) {
+    if (n == 0) return;
+    split_kernel<HT><<<(unsigned)div_up(n, (size_t)256), 256, 0, st>>>(v, hi, lo, n);
+    EMIDA_LAUNCHED("split_kernel");
+}
+
+#define INST_MFMA(HT)                                                                                          \
+    template void launch_mfma_corr<HT>(const HT*, const HT*, const HT*, const HT*, float*, size2, size2, uint32_t, \
+                                       uint32_t, hipStream_t);                                                 \
+    template void launch_split<HT>(const float*, HT*, HT*, size_t, hipStream_t);                               \
+    template void launch_prepare_split<uint16_t, uint32_t, HT>(const uint16_t*, HT*, HT*, const float*,        \
+                                                               const float*, const uint32_t*, const size2*, size2, \
+                                                               size2, uint32_t, uint32_t, hipStream_t);        \
+    template void launch_prepare_split<float, float, HT>(const float*, HT*, HT*, const float*, const float*,    \
+                                                         const float*, const size2*, size2, size2, uint32_t,    \
+                                                         uint32_t, hipStream_t);                               \
+    template void launch_prepare_split<double, double, HT>(const double*, HT*, HT*, const float*, const float*, \
+                                                           const double*, const size2*, size2, size2, uint32_t, \
+                                                           uint32_t, hipStream_t);
+INST_MFMA(__bf16)
+INST_MFMA(_Float16)
+
+}  // namespace emida_amd

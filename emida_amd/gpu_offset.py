@@ -15,6 +15,8 @@ from . import _capi as C
 
 CROSS_POLICY_BRUTE = C.CROSS_POLICY_BRUTE
 CROSS_POLICY_FFT = C.CROSS_POLICY_FFT
+CROSS_POLICY_BRUTE_MFMA = C.CROSS_POLICY_BRUTE_MFMA
+MFMA_BF16, MFMA_BF16_SPLIT, MFMA_F16, MFMA_F16_SPLIT = C.MFMA_BF16, C.MFMA_BF16_SPLIT, C.MFMA_F16, C.MFMA_F16_SPLIT
 
 
 def data_index_dtype(t):
@@ -94,12 +96,14 @@ class GpuOffset:
 
     T: np.float32 or np.float64 (correlation precision); IN: np.uint16, np.float32
     or np.float64 (pixel type); begins: list of (x, y) ROI corners (copied);
-    slice_size, cross_size, src_size: (x, y); policy: CROSS_POLICY_BRUTE or
-    CROSS_POLICY_FFT; s: fitsize 3, 5, 7 or 9.
+    slice_size, cross_size, src_size: (x, y); policy: CROSS_POLICY_BRUTE,
+    CROSS_POLICY_FFT or CROSS_POLICY_BRUTE_MFMA (matrix-core brute correlation,
+    T must be float32); s: fitsize 3, 5, 7 or 9; mfma_mode: MFMA_BF16,
+    MFMA_BF16_SPLIT, MFMA_F16 or MFMA_F16_SPLIT (used by CROSS_POLICY_BRUTE_MFMA).
     """
 
     def __init__(self, T, IN, src_size, begins, slice_size, cross_size, batch_size, policy=CROSS_POLICY_BRUTE,
-                 s=3, in_buffers_size=1):
+                 s=3, in_buffers_size=1, mfma_mode=MFMA_BF16_SPLIT):
         self.T = np.dtype(T)
         self.IN = np.dtype(IN)
         self.src_size = tuple(int(v) for v in src_size)
@@ -115,6 +119,9 @@ class GpuOffset:
                                             C.size2(cross_size), self.batch_size, self.policy, self.s,
                                             int(in_buffers_size)))
         self._h = h
+        if self.policy == CROSS_POLICY_BRUTE_MFMA:
+            C.check(C.lib().emida_offset_set_mfma_mode(h, int(mfma_mode)))
+        self.mfma_mode = int(mfma_mode)
         info = C.OffsetInfo()
         C.check(C.lib().emida_offset_get_info(h, ctypes.byref(info)))
         self.total_slices = info.total_slices

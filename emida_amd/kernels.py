@@ -66,6 +66,20 @@ def run_cross_corr(pics, ref, size, res_size, ref_slices=1, batch_size=1):
     return out.download(pics.dtype)
 
 
+def run_cross_corr_mfma(pics, ref, size, res_size, ref_slices=1, batch_size=1, mode=C.MFMA_BF16_SPLIT):
+    """Matrix-core brute correlation of float32 prepared slices: same layout and
+    meaning as run_cross_corr, products of bf16/fp16 halves accumulated in fp32
+    (mode MFMA_*_SPLIT: hi*hi + hi*lo + lo*hi)."""
+    pics = np.ascontiguousarray(pics, dtype=np.float32)
+    ref = np.ascontiguousarray(ref, dtype=np.float32)
+    dp, dr = DeviceBuffer.from_array(pics), DeviceBuffer.from_array(ref)
+    n = res_size[0] * res_size[1] * ref_slices * batch_size
+    out = DeviceBuffer(n * 4)
+    C.check(C.lib().emida_run_cross_corr_mfma(int(mode), dp.ptr, dr.ptr, out.ptr, C.size2(size), C.size2(res_size),
+                                              ref_slices, batch_size, None))
+    return out.download(np.float32)
+
+
 def run_cross_corr_fft(pics, ref, size, ref_slices=1, batch_size=1):
     """cross_corr_fft (src/gpu_offset.hpp:303-316) on prepared slices: returns the
     (2s)^2 circular correlation buffers, unnormalised."""

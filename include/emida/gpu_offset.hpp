@@ -55,7 +55,12 @@ inline bool operator==(const vec2<T>& a, const vec2<T>& b) {
 
 using size2_t = vec2<esize_t>;
 
-enum cross_policy { CROSS_POLICY_BRUTE = EMIDA_CROSS_POLICY_BRUTE, CROSS_POLICY_FFT = EMIDA_CROSS_POLICY_FFT };
+enum cross_policy {
+    CROSS_POLICY_BRUTE = EMIDA_CROSS_POLICY_BRUTE,
+    CROSS_POLICY_FFT = EMIDA_CROSS_POLICY_FFT,
+    // Matrix-core brute correlation (no reference counterpart; T = float only).
+    CROSS_POLICY_BRUTE_MFMA = EMIDA_CROSS_POLICY_BRUTE_MFMA
+};
 
 template <typename T>
 struct offsets_t {  // src/subpixel_max.hpp:69-74
@@ -115,6 +120,7 @@ public:
         emida_offset_info info;
         detail::check(emida_offset_get_info(h_, &info), "gpu_offset");
         total_ = info.total_slices;
+
         neigh_ = info.neigh_size;
     }
     gpu_offset(const gpu_offset&) = delete;
@@ -132,6 +138,11 @@ public:
 
     esize_t total_slices() const { return total_; }
     esize_t neigh_size() const { return neigh_; }
+
+    // Half type of CROSS_POLICY_BRUTE_MFMA; call before allocate_memory (no reference counterpart).
+    void set_mfma_mode(emida_mfma_mode mode) {
+        detail::check(emida_offset_set_mfma_mode(h_, mode), "set_mfma_mode");
+    }
 
     // src/gpu_offset.hpp:127
     void allocate_memory(IN* temp) { detail::check(emida_offset_allocate_memory(h_, temp), "allocate_memory"); }

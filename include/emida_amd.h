@@ -39,8 +39,24 @@ extern "C" {
  * type) is U16, F32 or F64; I32 is accepted by emida_run_cross_corr only. */
 typedef enum { EMIDA_F32 = 0, EMIDA_F64 = 1, EMIDA_I32 = 2, EMIDA_U16 = 3 } emida_type;
 
-/* cross_policy, src/common.hpp:141-145 */
-typedef enum { EMIDA_CROSS_POLICY_BRUTE = 0, EMIDA_CROSS_POLICY_FFT = 1 } emida_cross_policy;
+/* cross_policy, src/common.hpp:141-145.  BRUTE_MFMA (no reference counterpart) is
+ * the brute-force correlation as an implicit GEMM on BF16/FP16 matrix cores with
+ * FP32 accumulation (precision set by emida_offset_set_mfma_mode); it requires
+ * T = F32 and a slice width that is a multiple of 8 up to 128. */
+typedef enum {
+    EMIDA_CROSS_POLICY_BRUTE = 0,
+    EMIDA_CROSS_POLICY_FFT = 1,
+    EMIDA_CROSS_POLICY_BRUTE_MFMA = 2
+} emida_cross_policy;
+
+/* Operand precision of EMIDA_CROSS_POLICY_BRUTE_MFMA.  *_SPLIT modes store every
+ * prepared value as hi + lo halves and accumulate hi*hi + hi*lo + lo*hi. */
+typedef enum {
+    EMIDA_MFMA_BF16 = 0,
+    EMIDA_MFMA_BF16_SPLIT = 1,
+    EMIDA_MFMA_F16 = 2,
+    EMIDA_MFMA_F16_SPLIT = 3
+} emida_mfma_mode;
 
 enum {
     EMIDA_OK = 0,
@@ -132,6 +148,10 @@ int emida_offset_get_offset(emida_offset* h, const void* pic, double* offsets, d
 int emida_offset_process_device(emida_offset* h, const void* d_pics, uint32_t n_patterns, void* d_maxes,
                                 void* d_neighbors, double* d_offsets, double* d_coefs, void* stream);
 
+/* Operand precision of the BRUTE_MFMA policy (default EMIDA_MFMA_BF16_SPLIT); call
+ * before emida_offset_allocate_memory.  No reference counterpart. */
+int emida_offset_set_mfma_mode(emida_offset* h, emida_mfma_mode mode);
+
 /* Kernel timing with HIP events on the stream the kernels run on.  Returns the
  * summed time and launch count of the dominant kernel (the FFT pipeline kernel
  * for FFT, the correlation kernel for brute) since timing was enabled. */
@@ -182,6 +202,14 @@ int emida_run_cross_corr(emida_type T, const void* pics, const void* ref, void* 
  * side <= 64. */
 int emida_run_cross_corr_fft(emida_type T, const void* pics, const void* ref, void* res, emida_size2 size,
                              uint32_t ref_slices, uint32_t batch_size, void* stream);
+
+/* Brute-force correlation of prepared float slices on matrix cores (the computation
+ * of run_cross_corr, src/cross_corr.cu:68-74, in the given operand precision, FP32
+ * accumulation, float results).  No reference counterpart.  size.x a multiple of 8
+ * up to 128; res_size <= 2*size-1. */
+int emida_run_cross_corr_mfma(emida_mfma_mode mode, const float* pics, const float* ref, float* res,
+                              emida_size2 size, emida_size2 res_size, uint32_t ref_slices, uint32_t batch_size,
+                              void* stream);
 
 /* run_hadamard(pics, ref, one_size, ref_slices, batch_size), src/kernels.cuh:122-123,
  * src/hadamard.cu:43-49: pics[i] = pics[i] * conj(ref[i mod (one_size.area * ref_slices)])

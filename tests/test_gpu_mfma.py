@@ -1,0 +1,167 @@
+"""Matrix-core brute correlation (the precision form) on the GPU, through the C ABI.
+
+The reference has one brute-force correlation, src/cross_corr.cu:11-66, in T = float
+or double.  The matrix-core form computes the same sums from bf16/fp16 halves of the
+prepared float values with fp32 accumulation, optionally as hi*hi + hi*lo + lo*hi.
+Checks:
+  * integer-valued inputs are exact in every half type and every partial sum stays
+    below 2^24, so the result must equal the FP64 oracle bit for bit - this pins the
+    operand layout, the Toeplitz indexing and the output order;
+  * random inputs: every element within the worst-case rounding bound of its mode,
+    and the split modes far closer than the plain ones;
+  * whole pipeline: offsets against the FP64 oracle (brute, double), with the
+    maximum error stated per mode.
+"""
+import numpy as np
+import pytest
+
+import emida_amd
+from emida_amd import kernels as K
+from emida_amd import synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+MODES = {  # mode: (unit roundoff of the half type, split, subnormal half-spacing)
+    emida_amd.MFMA_BF16: (2.0 ** -8, False, 0.0),
+    emida_amd.MFMA_BF16_SPLIT: (2.0 ** -8, True, 0.0),
+    emida_amd.MFMA_F16: (2.0 ** -11, False, 2.0 ** -25),
+    emida_amd.MFMA_F16_SPLIT: (2.0 ** -11, True, 2.0 ** -25),
+}
+SHAPES = [  # slice (x, y), cross (x, y), ROIs, patterns
+    ((128, 128), (255, 255), 2, 3),
+    ((64, 64), (127, 127), 3, 33),
+    ((40, 24), (33, 17), 2, 5),
+    ((8, 8), (15, 15), 1, 2),
+    ((16, 40), (31, 79), 2, 3),
+    ((24, 16), (32, 20), 2, 4),
+]
+
+
+def _oracle(p, r, size, cross, nb, npat):
+    return O.cross_corr(p.astype(np.float64), r.astype(np.float64), size, cross, nb, npat)
+
+
+class RoundingBound:
+    """Worst case |mfma - exact| per element.  Operands: |v - hi| <= u|v| (+ the
+    subnormal half-spacing h for fp16); split: |v - hi - lo| <= u^2|v| + h, and the
+    omitted lo*lo term is <= u^2(1+u)^2|v||r|.  Products of halves are exact in fp32;
+    the fp32 sum of n terms adds gamma_n = n 2^-24 / (1 - n 2^-24) of sum|terms|."""
+
+    def __init__(self, p, r, size, cross, nb, npat):
+        ap, ar = np.abs(p).astype(np.float64), np.abs(r).astype(np.float64)
+        self.n = size[0] * size[1]
+        self.pr = _oracle(ap, ar, size, cross, nb, npat)
+        self.p1 = _oracle(ap, np.ones_like(ar), size, cross, nb, npat)
+        self.r1 = _oracle(np.ones_like(ap), ar, size, cross, nb, npat)
+
+    def __call__(self, mode):
+        u, split, h = MODES[mode]
+        n = self.n * (3 if split else 1)
+        gamma = n * 2.0 ** -24 / (1 - n * 2.0 ** -24)
+        rel = (3.1 * u * u if split else 2 * u + u * u) * (1 + gamma) + gamma * (1 + u) ** 2
+        return rel * self.pr + 1.01 * h * (self.p1 + self.r1) + 1e-30
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+@pytest.mark.parametrize("size,cross,nb,npat", SHAPES)
+def test_integer_inputs_bitexact(size, cross, nb, npat, mode):
+    rng = np.random.default_rng(size[0] * 7 + size[1] + npat)
+    p = rng.integers(-4, 5, size=size[0] * size[1] * nb * npat).astype(np.float32)
+    r = rng.integers(-4, 5, size=size[0] * size[1] * nb).astype(np.float32)
+    got = K.run_cross_corr_mfma(p, r, size, cross, nb, npat, mode)
+    np.testing.assert_array_equal(got.astype(np.float64), _oracle(p, r, size, cross, nb, npat))
+
+
+@pytest.mark.parametrize("size,cross,nb,npat", SHAPES[:3])
+def test_random_inputs_within_rounding_bound(size, cross, nb, npat):
+    rng = np.random.default_rng(5)
+    p = (rng.standard_normal(size[0] * size[1] * nb * npat) * 40).astype(np.float32)
+    r = (rng.standard_normal(size[0] * size[1] * nb) * 40).astype(np.float32)
+    exact = _oracle(p, r, size, cross, nb, npat)
+    rb = RoundingBound(p, r, size, cross, nb, npat)
+    worst = {}
+    for mode in MODES:
+        got = K.run_cross_corr_mfma(p, r, size, cross, nb, npat, mode).astype(np.float64)
+        err = np.abs(got - exact)
+        bound = rb(mode)
+        assert (err <= bound).all(), (mode, float((err / bound).max()))
+        worst[mode] = err.max()
+    # hi + lo products recover most of what the single half type loses
+    assert worst[emida_amd.MFMA_BF16_SPLIT] * 16 < worst[emida_amd.MFMA_BF16]
+    assert worst[emida_amd.MFMA_F16_SPLIT] * 16 < worst[emida_amd.MFMA_F16]
+    print("max abs error per mode", {k: float(v) for k, v in worst.items()})
+
+
+def test_rejects_unsupported_shapes():
+    p = np.zeros(60 * 60, np.float32)
+    with pytest.raises(emida_amd.EmidaError) as e:
+        K.run_cross_corr_mfma(p, p, (60, 60), (119, 119))  # width not a multiple of 8
+    assert e.value.code == emida_amd._capi.ERR_UNSUPPORTED
+    p = np.zeros(136 * 8, np.float32)
+    with pytest.raises(emida_amd.EmidaError):
+        K.run_cross_corr_mfma(p, p, (136, 8), (15, 15))  # wider than 128
+    p = np.zeros(16 * 16, np.float32)
+    with pytest.raises(emida_amd.EmidaError) as e:
+        K.run_cross_corr_mfma(p, p, (16, 16), (33, 31))  # cross > 2s - 1
+    assert e.value.code == emida_amd._capi.ERR_ARG
+
+
+# ------------------------------------------------------------------ whole pipeline
+def well_posed(coefs, fit):
+    """ROIs whose FP64 quadratic fit has its maximum inside the F x F neighbourhood;
+    elsewhere the fitted position is an extrapolation, ill-conditioned in any precision."""
+    b, c, d, e, f = (coefs[:, i] for i in range(1, 6))
+    det = 4 * f * d - e * e
+    with np.errstate(divide="ignore", invalid="ignore"):
+        py = (b * e - 2 * c * d) / det
+        px = (-e * py - b) / (2 * d)
+    return (d < 0) & (det > 0) & (px >= 0) & (px <= fit - 1) & (py >= 0) & (py <= fit - 1)
+
+
+def _mfma_offsets(ps, begins, slice_size, cross, fit, batch, mode):
+    with emida_amd.GpuOffset(np.float32, np.uint16, ps.size, begins, slice_size, cross, batch,
+                             emida_amd.CROSS_POLICY_BRUTE_MFMA, fit, mfma_mode=mode) as g:
+        g.allocate_memory(ps.ref)
+        maxes, neigh = g.new_result_buffers(pinned=False)
+        idx = g.transfer_pic_to_device_async(ps.pics[:batch])
+        g.wait_uploads()
+        g.get_offset_core(maxes, neigh, idx)
+        offs, _ = g.finalize(maxes, neigh)
+        p_offs, _ = g.get_offset(ps.pics[:batch])
+        np.testing.assert_array_equal(offs, p_offs)  # both host forms, same results
+        return offs
+
+
+# max offset error vs FP64 on well-posed ROIs (measured on MI355X: bf16 2.4e-3,
+# bf16 split 3.5e-5, fp16 2.8e-4, fp16 split 2.9e-5 px); every mode meets 0.01 px
+PIPE_TOL = {emida_amd.MFMA_BF16: 0.01, emida_amd.MFMA_BF16_SPLIT: 5e-4, emida_amd.MFMA_F16: 2e-3,
+            emida_amd.MFMA_F16_SPLIT: 5e-4}
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+def test_pipeline_vs_fp64_oracle(mode):
+    ps = synth.PatternSet("mfma", 256, 3, t_range=2.0)
+    begins = synth.grid_begins(256, 4, 32)
+    eo, ec = O.get_offset(ps.pics.ravel(), ps.ref, ps.size, begins, (64, 64), (127, 127), 3, 7, O.POLICY_BRUTE,
+                          np.float64)
+    offs = _mfma_offsets(ps, begins, (64, 64), (127, 127), 7, 3, mode)
+    ok = well_posed(ec, 7)
+    assert ok.mean() > 0.5
+    err = np.abs(offs - eo)[ok].max()
+    print(f"mode {mode}: max error {err:.2e} px over {ok.sum()} well-posed ROIs")
+    assert err < PIPE_TOL[mode]
+
+
+@pytest.mark.slow
+def test_precision_form_workload_subset_vs_fp64_oracle():
+    # the precision-form workload's shape: 873 x 873 patterns, ROIs of 128 x 128 on a
+    # 10 x 10 grid, full 255 x 255 correlation, fitsize 7 (25 of the 100 ROIs)
+    ps = synth.PatternSet("mfma873", 873, 1, t_range=3.0)
+    begins = synth.grid_begins(873, 10, 64)[::4]
+    eo, ec = O.get_offset(ps.pics.ravel(), ps.ref, ps.size, begins, (128, 128), (255, 255), 1, 7, O.POLICY_BRUTE,
+                          np.float64)
+    ok = well_posed(ec, 7)
+    for mode in (emida_amd.MFMA_BF16_SPLIT, emida_amd.MFMA_F16_SPLIT):
+        offs = _mfma_offsets(ps, begins, (128, 128), (255, 255), 7, 1, mode)
+        assert np.abs(offs - eo)[ok].max() < 0.01

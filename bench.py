@@ -37,6 +37,8 @@ METRIC = "patterns/s + ROI offsets/s at ≤0.01 px err vs FP64; p50/p99 pattern 
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+MFMA_16BIT_PEAK_TFLOPS = 2500.0  # dense bf16 / fp16
+MFMA_MODES = {"bf16": (0, 1), "bf16_split": (1, 3), "f16": (2, 1), "f16_split": (3, 3)}  # code, products
 
 
 def fft_flops_per_roi(m):
@@ -118,6 +120,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--unique", type=int, default=64, help="distinct synthetic patterns (tiled to the job size)")
     ap.add_argument("--copy-calibration-mb", type=float, default=0, help="only run a device copy of this size")
+    ap.add_argument("--mfma", choices=sorted(MFMA_MODES), default=None,
+                    help="brute workloads: matrix-core correlation in this mode instead of the plain form")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -137,6 +141,12 @@ def main():
     cross = (4 * r - 1, 4 * r - 1)
     T = np.float32 if cfg["dtype"] == "f32" else np.float64
     policy = emida_amd.CROSS_POLICY_FFT if cfg["policy"] == "fft" else emida_amd.CROSS_POLICY_BRUTE
+    mfma_code, mfma_products = MFMA_MODES[args.mfma] if args.mfma else (0, 1)
+    if args.mfma:
+        if cfg["policy"] != "brute":
+            print("bench.py: --mfma applies to brute workloads only", file=sys.stderr)
+            return 2
+        T, policy = np.float32, emida_amd.CROSS_POLICY_BRUTE_MFMA
     batch, n_pat = cfg["batch"], cfg["patterns"]
     n_batches = n_pat // batch
 
@@ -149,7 +159,8 @@ def main():
     print(f"# {dev_name}, {cus} CUs; generated {args.unique} distinct patterns in {t_gen:.1f}s", file=sys.stderr,
           flush=True)
 
-    g = emida_amd.GpuOffset(T, np.uint16, ps.size, begins, slice_size, cross, batch, policy, cfg["fit"])
+    g = emida_amd.GpuOffset(T, np.uint16, ps.size, begins, slice_size, cross, batch, policy, cfg["fit"],
+                            mfma_mode=mfma_code)
     g.allocate_memory(ps.ref)
     d_pics = emida_amd.DeviceBuffer.from_array(ps.pics)  # the whole job, resident in HBM
     pat_bytes = ps.size[0] * ps.size[1] * 2
@@ -199,9 +210,14 @@ def main():
     if cfg["policy"] == "fft":
         flops = fft_flops_per_roi(4 * r) * total
         peak, pipe = FP32_PEAK_TFLOPS if T == np.float32 else FP64_PEAK_TFLOPS, "FP32 vector" if T == np.float32 else "FP64 vector"
+    elif args.mfma:
+        # matrix-core work: every product of the split form is issued (3 for hi+lo)
+        flops = brute_flops_per_roi(slice_size, cross) * total * mfma_products
+        peak, pipe = MFMA_16BIT_PEAK_TFLOPS, f"{args.mfma.split('_')[0]} MFMA, {mfma_products} product(s)"
     else:
         flops = brute_flops_per_roi(slice_size, cross) * total
-        peak, pipe = (FP32_PEAK_TFLOPS if T == np.float32 else FP64_PEAK_TFLOPS), "vector FMA"
+        peak, pipe = ((FP32_PEAK_TFLOPS, "FP32 vector FMA") if T == np.float32 else
+                      (FP64_PEAK_TFLOPS, "FP64 vector FMA"))
     achieved = flops / (k_avg_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "pipe": pipe, "kernel": kname, "achieved": round(achieved, 3), "peak": peak,
                 "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": read_traffic(args.workload),
@@ -211,11 +227,13 @@ def main():
 
     out = {"metric": METRIC, "value": round(pats_per_s, 2), "unit": "patterns/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32" if T == np.float32
-           else "f64", "data": "synthetic",
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": (f"{args.mfma} (f32 accumulate)" if args.mfma else "f32" if T == np.float32 else "f64"),
+           "data": "synthetic",
            "config": {"workload": args.workload, "pattern": f"{cfg['size']}x{cfg['size']} uint16",
                       "patterns": n_pat, "rois": n_roi, "roi": f"{2 * r}x{2 * r}",
-                      "policy": cfg["policy"], "fitsize": cfg["fit"], "batch": batch,
+                      "policy": cfg["policy"] + (f"+mfma:{args.mfma}" if args.mfma else ""),
+                      "fitsize": cfg["fit"], "batch": batch,
                       "distinct_patterns": args.unique, "device": dev_name},
            "roi_offsets_per_s": round(pats_per_s * n_roi, 1),
            "p50_pattern_latency_ms": round(float(np.percentile(lat, 50)), 4),

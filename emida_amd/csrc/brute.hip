@@ -6,9 +6,11 @@
 // inner, as fused multiply-adds (the reference's nvcc build contracts
 // `sum += a * b`), so double and float results are bit-identical to that order.
 //
-// Work decomposition: one workgroup per (slice, 4-row band of shifts, 64-column
-// band of shifts), one thread per shift; the reference and pattern slices are
-// staged in LDS when both fit in 64 KiB, and the overlap loop runs over LDS.
+// Two kernels: a register-blocked one (below, "tiled") for slices up to 128 pixels
+// wide and correlation windows up to 256 wide, and a thread-per-shift one for any
+// other shape: one workgroup per (slice, 4-row band of shifts, 64-column band of
+// shifts); the reference and pattern slices are staged in LDS when both fit in
+// 64 KiB, and the overlap loop runs over LDS.
 #include "kernels.hpp"
 
 namespace emida_amd {
@@ -62,6 +64,176 @@ __global__ __launch_bounds__(kBruteTX* kBruteTY) void cross_corr_kernel(const T*
     res[(size_t)sn * res_size.area() + (size_t)ry * res_size.x + rx] = sum;
 }
 
+// ------------------------------------------------------------------ tiled kernel
+// The same sums, register-blocked.  A workgroup takes 16 patterns of one ROI, a band
+// of x-shifts (64 per wave) and a band of y-shifts (4 per wave); it walks the
+// reference rows y and, per row, the pattern columns k in blocks of 8.  A thread
+// holds 2 patterns x 8 consecutive x-shifts x 4 y-shifts of accumulators; per column
+// k it reads 2 x 4 pattern values from LDS and slides an 8-wide window over the
+// reference row, so each pattern value feeds 8 and each reference value 8 x 4
+// multiply-adds.  The reference row is zero-padded in LDS and pattern rows are
+// zero-padded to a multiple of 8, so the loop runs over the union of the thread's
+// overlaps: a term outside a shift's overlap adds a product with an exact zero and
+// leaves its sum unchanged (for finite values), and every sum still sees its own
+// terms y outer, x inner, in the reference's order.
+constexpr int kTiledP = 16;   // patterns per workgroup
+constexpr int kTiledMI = 2;   // patterns per thread
+constexpr int kTiledNI = 8;   // x-shifts per thread
+constexpr int kTiledTT = 4;   // y-shifts per thread
+constexpr int kTiledSXW = 64; // x-shifts per wave
+constexpr size_t kTiledRingMax = 64 * 1024;
+
+struct tiled_shape {
+    int wx, wy;       // waves along x-shifts and y-shifts
+    int rows;         // pattern rows per step (y-shifts per workgroup) = 4 wy
+    int spad;         // slice width rounded up to 8
+    int off;          // reference row offset in the padded LDS row
+    int rlen;         // padded reference row length
+    size_t lds;       // bytes: ring + reference row
+};
+
+template <class T>
+inline bool tiled_plan(size2 size, size2 res, tiled_shape& t) {
+    t.spad = (int)div_up(size.x, 8u) * 8;
+    if (t.spad > 128 || res.x > 4u * kTiledSXW) return false;
+    t.wx = (int)div_up(res.x, (uint32_t)kTiledSXW);
+    t.wy = t.wx == 1 ? 4 : (t.wx == 2 ? 2 : 1);
+    while (t.wy > 1 && (size_t)kTiledP * kTiledTT * t.wy * t.spad * sizeof(T) > kTiledRingMax) t.wy /= 2;
+    t.rows = kTiledTT * t.wy;
+    // a thread's window reads R[y][k - sx] for k in [k8 - 8, khi) and its x-shifts
+    // sx in [-rx, 64 wx - rx); with off = 64 wx - rx + 16 every index k - sx + off lies
+    // in [16, spad + rx + off)
+    const int rx = (int)(res.x - 1) / 2;
+    t.off = kTiledSXW * t.wx - rx + 16;
+    t.rlen = t.spad + rx + t.off;
+    t.lds = (size_t)kTiledP * t.rows * t.spad * sizeof(T) + (size_t)t.rlen * sizeof(T);
+    return true;
+}
+
+template <class T>
+struct pair_of {
+    T v[2];
+};
+
+// Pattern ring: [rows][spad][16] with the 2-pattern pair index of column k stored at
+// pair ^ (k & 7) (conflict-free transposed writes and pair reads).
+template <class T>
+__global__ __launch_bounds__(256, 2) void cross_corr_tiled_kernel(const T* __restrict__ pics,
+                                                                   const T* __restrict__ ref, T* __restrict__ res,
+                                                                   size2 size, size2 res_size, uint32_t nb,
+                                                                   uint32_t npat, tiled_shape sh, uint32_t n_wg) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* ring = reinterpret_cast<T*>(smem_raw);
+    T* rrow = ring + (size_t)kTiledP * sh.rows * sh.spad;
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid % 64, wave = tid / 64;
+    const int S = (int)size.x, SY = (int)size.y, SP = sh.spad, ROWS = sh.rows;
+    const uint32_t n_syt = div_up(res_size.y, (uint32_t)ROWS);
+    const uint32_t L = xcd_remap(blockIdx.x, n_wg);
+    const uint32_t syt = L % n_syt, slab = L / n_syt;
+    const uint32_t roi = slab % nb, p0 = (slab / nb) * kTiledP;
+    const int rx = (int)(res_size.x - 1) / 2, ry = (int)(res_size.y - 1) / 2;
+    const int sy0 = (int)syt * ROWS - ry;
+    const size_t slice_elems = (size_t)size.x * size.y;
+    const T* refs = ref + (size_t)roi * slice_elems;
+
+    const int wx = (wave % sh.wx + (int)syt) % sh.wx, wy = wave / sh.wx;
+    const int pg = lane % 8, sg = lane / 8;
+    const int sxw0 = -rx + wx * kTiledSXW;          // first x-shift of the wave
+    const int sxb = sxw0 + sg * kTiledNI;           // first x-shift of the thread
+    const int syt0 = sy0 + wy * kTiledTT;           // first y-shift of the thread
+
+    T acc[kTiledMI][kTiledNI][kTiledTT];
+#pragma unroll
+    for (int i = 0; i < kTiledMI; ++i)
+#pragma unroll
+        for (int n = 0; n < kTiledNI; ++n)
+#pragma unroll
+            for (int t = 0; t < kTiledTT; ++t) acc[i][n][t] = T(0);
+
+    for (size_t e = tid; e * sizeof(T) < sh.lds; e += nt) ring[e] = T(0);
+    __syncthreads();
+
+    auto stage_prow = [&](int Y) {
+        if (Y < 0 || Y >= SY) return;
+        T* dst = ring + (size_t)(Y % ROWS) * SP * kTiledP;
+        for (int e = tid; e < kTiledP * S; e += nt) {
+            const int p = e / S, k = e % S;
+            const uint32_t pp = p0 + p;
+            const T v = pp < npat ? pics[((size_t)pp * nb + roi) * slice_elems + (size_t)Y * S + k] : T(0);
+            dst[k * kTiledP + 2 * ((p / 2) ^ (k & 7)) + (p & 1)] = v;
+        }
+    };
+    auto stage_rrow = [&](int y) {
+        for (int e = tid; e < S; e += nt) rrow[sh.off + e] = refs[(size_t)y * S + e];
+    };
+
+    // k range of the wave: union of the overlaps of its x-shifts, in blocks of 8
+    const int klo = max(0, sxw0) & ~7;
+    const int khi = min(SP, (min(S, S + sxw0 + kTiledSXW - 1) + 7) & ~7);
+
+    const int y_lo = max(0, -(sy0 + ROWS - 1)), y_hi = min(SY, SY - sy0);
+    for (int Y = y_lo + sy0; Y < y_lo + sy0 + ROWS - 1; ++Y) stage_prow(Y);
+    for (int y = y_lo; y < y_hi; ++y) {
+        stage_prow(y + sy0 + ROWS - 1);
+        stage_rrow(y);
+        __syncthreads();
+        if (klo < khi) {
+            // window values: cur[j] = R[y][k8 + j - sxb], prev[j] = R[y][k8 - 8 + j - sxb]
+            T prev[8], cur[8];
+            const T* rp = rrow + sh.off - sxb;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) cur[j] = rp[klo - 8 + j];
+            for (int k8 = klo; k8 < khi; k8 += 8) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    prev[j] = cur[j];
+                    cur[j] = rp[k8 + j];
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int k = k8 + j;
+                    pair_of<T> a[kTiledTT];
+#pragma unroll
+                    for (int t = 0; t < kTiledTT; ++t) {
+                        const int Y = y + syt0 + t;
+                        const int slot = (Y % ROWS + ROWS) % ROWS;
+                        a[t] = *reinterpret_cast<const pair_of<T>*>(ring + ((size_t)slot * SP + k) * kTiledP +
+                                                                    2 * (pg ^ (k & 7)));
+                    }
+#pragma unroll
+                    for (int t = 0; t < kTiledTT; ++t) {
+                        const int Y = y + syt0 + t;
+                        if (Y < 0 || Y >= SY) continue;
+#pragma unroll
+                        for (int n = 0; n < kTiledNI; ++n) {
+                            const T b = (j - n >= 0) ? cur[(j - n) & 7] : prev[(8 + j - n) & 7];
+#pragma unroll
+                            for (int i = 0; i < kTiledMI; ++i) acc[i][n][t] = mac<T>(a[t].v[i], b, acc[i][n][t]);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int t = 0; t < kTiledTT; ++t) {
+        const int sy = syt0 + t;
+        if (sy + ry >= (int)res_size.y) continue;
+#pragma unroll
+        for (int i = 0; i < kTiledMI; ++i) {
+            const uint32_t p = p0 + 2 * pg + i;
+            if (p >= npat) continue;
+            T* out = res + (((size_t)p * nb + roi) * res_size.y + (sy + ry)) * res_size.x;
+#pragma unroll
+            for (int n = 0; n < kTiledNI; ++n) {
+                const int sx = sxb + n;
+                if (sx + rx < (int)res_size.x) out[sx + rx] = acc[i][n][t];
+            }
+        }
+    }
+}
+
 template <class T>
 void launch_cross_corr(const T* pics, const T* ref, T* res, size2 size, size2 res_size, uint32_t ref_slices,
                        uint32_t batch, hipStream_t st) {
@@ -70,6 +242,22 @@ void launch_cross_corr(const T* pics, const T* ref, T* res, size2 size, size2 re
     const uint32_t total = ref_slices * batch;
     if (total == 0 || res_size.area() == 0) return;
     if (total > 65535u) throw arg_error("cross_corr: more than 65535 slices in one launch");
+    tiled_shape sh;
+    if (tiled_plan<T>(size, res_size, sh)) {
+        const uint64_t n_wg = (uint64_t)ref_slices * div_up(batch, (uint32_t)kTiledP) *
+                              div_up(res_size.y, (uint32_t)sh.rows);
+        // below one workgroup per CU the thread-per-shift kernel's finer grid is faster
+        if (n_wg >= 256 && n_wg <= (1u << 24)) {
+            auto kern = cross_corr_tiled_kernel<T>;
+            static const hipError_t attr =
+                hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+            EMIDA_HIP(attr);
+            kern<<<(uint32_t)n_wg, 64 * sh.wx * sh.wy, sh.lds, st>>>(pics, ref, res, size, res_size, ref_slices,
+                                                                       batch, sh, (uint32_t)n_wg);
+            EMIDA_LAUNCHED("cross_corr_tiled_kernel");
+            return;
+        }
+    }
     dim3 block(kBruteTX, kBruteTY);
     dim3 grid(div_up(res_size.x, (uint32_t)kBruteTX), div_up(res_size.y, (uint32_t)kBruteTY), total);
     const size_t lds = 2 * (size_t)size.area() * sizeof(T);

@@ -57,6 +57,16 @@ inline void hip_check(hipError_t e, const char* what) {
 #define EMIDA_HIP(call) ::emida_amd::hip_check((call), #call)
 #define EMIDA_LAUNCHED(name) ::emida_amd::hip_check(hipGetLastError(), name)
 
+// XCD-aware workgroup order.  Workgroup b of a 1-D grid of n runs on XCD b mod 8;
+// this returns a logical index such that XCD x executes the contiguous run
+// [x*(n/8) + min(x, n%8), ...) of logical indices, so that neighbouring logical
+// indices (which share input data) share one XCD's L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
+    constexpr uint32_t X = 8;
+    const uint32_t x = b % X, l = b / X, per = n / X, rem = n % X;
+    return x * per + (x < rem ? x : rem) + l;
+}
+
 // Position policies of the correlation buffer (reference src/kernels.cuh:22-53).
 // Identity: buffer index = y * size.x + x.  FFT: the circular (2s) buffer is
 // addressed through the shift (p + s + 1) mod 2s.

@@ -10,12 +10,14 @@
 // the row).  All-zero Toeplitz tiles are skipped.
 //
 // Work decomposition: one workgroup = 32 patterns x every x-shift (4 waves x two
-// 32-wide tiles) x TS consecutive y-shifts; blockIdx.x = ROI + nb * pattern tile,
-// blockIdx.y = y-shift tile.  It walks the reference rows y; the
-// pattern rows Y = y + sy of its y-shifts live in an LDS ring (one new row per step),
-// and reference row y is staged as 8 shifted copies so that any 8-element window -
-// one lane's B fragment - is a single aligned 16-byte LDS read.  B fragments are
-// loaded once per (y, K tile) and reused for all TS y-shifts.
+// adjacent 32-wide tiles) x TS consecutive y-shifts, on a 1-D grid whose order keeps
+// the y-shift tiles of one (ROI, pattern tile) on one XCD.  It walks the reference
+// rows y; the pattern rows Y = y + sy of its y-shifts live in an LDS ring (one new
+// row per step), and reference row y is staged as 8 shifted copies so that any
+// 8-element window - one lane's B fragment - is a single aligned 16-byte LDS read.
+// Per K tile a wave reads each B fragment once for all TS y-shifts and each A
+// fragment once for both of its x-shift tiles.  The next step's rows are loaded
+// into registers before the matrix-core work of the current step.
 //
 // Matrix-core operand layout (v_mfma_f32_32x32x16_{bf16,f16}): lane l holds
 // A[l % 32][8 (l / 32) + t] and B[8 (l / 32) + t][l % 32] for t < 8; the 16
@@ -75,46 +77,66 @@ constexpr int kMfmaWaves = 4;
 constexpr int kMfmaTilesPerWave = 2;
 constexpr int kMfmaSX = kMfmaWaves * kMfmaTilesPerWave * 32;  // x-shifts per workgroup (256)
 constexpr int kMfmaM = 32;                                    // patterns per workgroup
+constexpr int kMfmaNT = kMfmaWaves * 64;
 
 template <class HT, bool SPLIT, int TS>
 struct mfma_plan {
-    static constexpr int KMAX = 128;          // slice width limit (K)
-    static constexpr int AROW = KMAX + 8;     // LDS row stride of a pattern row (elements)
-    static constexpr int RP = KMAX + kMfmaSX + 8;  // length of one shifted reference copy
-    static constexpr int NH = SPLIT ? 2 : 1;  // hi (+ lo) planes
-    static constexpr size_t lds_a = (size_t)NH * TS * kMfmaM * AROW * sizeof(HT);
+    static constexpr int KMAX = 128;                 // slice width limit (K); pattern row length in LDS
+    static constexpr int CHMAX = KMAX / 8;           // 16-byte chunks per pattern row
+    static constexpr int RP = KMAX + kMfmaSX + 8;    // length of one shifted reference copy
+    static constexpr int NH = SPLIT ? 2 : 1;         // hi (+ lo) planes
+    static constexpr size_t ring = (size_t)TS * kMfmaM * KMAX;  // elements of one plane of the row ring
+    static constexpr size_t lds_a = NH * ring * sizeof(HT);
     static constexpr size_t lds_r = (size_t)NH * 8 * RP * sizeof(HT);
     static constexpr size_t lds_bytes = lds_a + lds_r;
+    static constexpr int PF = (kMfmaM * CHMAX + kMfmaNT - 1) / kMfmaNT;  // prefetched row chunks per thread
 };
 
 // hi/lo: prepared pattern slices [slice][y][x] (slice = pattern * nb + roi);
 // rhi/rlo: prepared reference slices [roi][y][x]; corr: [slice][cross.y][cross.x].
+// n_wg = nb * ceil(npat / 32) * ceil(cross.y / TS) workgroups (1-D grid).
 template <class HT, bool SPLIT, int TS>
-__global__ __launch_bounds__(kMfmaWaves * 64) void mfma_corr_kernel(const HT* __restrict__ phi,
-                                                                    const HT* __restrict__ plo,
-                                                                    const HT* __restrict__ rhi,
-                                                                    const HT* __restrict__ rlo,
-                                                                    float* __restrict__ corr, size2 s, size2 cross,
-                                                                    uint32_t nb, uint32_t npat) {
+__global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restrict__ phi,
+                                                                const HT* __restrict__ plo,
+                                                                const HT* __restrict__ rhi,
+                                                                const HT* __restrict__ rlo,
+                                                                float* __restrict__ corr, size2 s, size2 cross,
+                                                                uint32_t nb, uint32_t npat, uint32_t n_wg) {
     using P = mfma_plan<HT, SPLIT, TS>;
     using V = typename mfma_vec<HT>::type;
-    constexpr int NT = kMfmaWaves * 64;
+    constexpr int NT = kMfmaNT;
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    HT* A = reinterpret_cast<HT*>(lds_raw);                 // [NH][TS][32][AROW]
-    HT* Rs = reinterpret_cast<HT*>(lds_raw + P::lds_a);     // [NH][8][RP]
+    // A: [NH][TS][32][KMAX] pattern-row ring, 16-byte chunk c of row m stored at chunk
+    //    c ^ (m & 15) (conflict-free fragment reads without padding);
+    // Rs: [NH][8][RP] shifted reference copies.
+    HT* A = reinterpret_cast<HT*>(lds_raw);
+    HT* Rs = reinterpret_cast<HT*>(lds_raw + P::lds_a);
 
     const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-    const uint32_t roi = blockIdx.x % nb, mt = blockIdx.x / nb;
+    const uint32_t n_syt = div_up(cross.y, (uint32_t)TS);
+    const uint32_t L = xcd_remap(blockIdx.x, n_wg);  // y-shift tiles of one (ROI, pattern tile) back to back
+    const uint32_t syt = L % n_syt, slab = L / n_syt;
+    const uint32_t roi = slab % nb, mt = slab / nb;
     const int rx = (int)(cross.x - 1) / 2, ry = (int)(cross.y - 1) / 2;
-    const int sy0 = (int)blockIdx.y * TS - ry;
-    const int sxg0 = -rx;       // first x-shift of the workgroup
-    const int off = 256 - rx;   // shifted-copy offset, see below
-    const int S = (int)s.x, SY = (int)s.y;
+    const int sy0 = (int)syt * TS - ry;
+    const int off = 256 - rx;  // shifted-copy offset, see below
+    const int S = (int)s.x, SY = (int)s.y, CH = S / 8;
     const int K16 = (S + 15) / 16 * 16;
     const uint32_t p0 = mt * kMfmaM;
     const size_t slice_elems = (size_t)s.x * s.y;
     const HT* refh = rhi + (size_t)roi * slice_elems;
     const HT* refl = SPLIT ? rlo + (size_t)roi * slice_elems : nullptr;
+
+    // this wave's two adjacent 32-wide x-shift tiles; the pair index rotates with the
+    // y-shift tile so that light (edge) and heavy (centre) pairs alternate per SIMD
+    const int pair = (wave + (int)syt) % kMfmaWaves;
+    int sx0[kMfmaTilesPerWave];
+    bool active[kMfmaTilesPerWave];
+#pragma unroll
+    for (int u = 0; u < kMfmaTilesPerWave; ++u) {
+        sx0[u] = -rx + (pair * kMfmaTilesPerWave + u) * 32;
+        active[u] = sx0[u] + rx < (int)cross.x;
+    }
 
     f32x16 acc[TS][kMfmaTilesPerWave];
 #pragma unroll
@@ -122,76 +144,126 @@ __global__ __launch_bounds__(kMfmaWaves * 64) void mfma_corr_kernel(const HT* __
 #pragma unroll
         for (int u = 0; u < kMfmaTilesPerWave; ++u) acc[t][u] = f32x16{0};
 
-    // stage pattern row Y of the 32 patterns into ring slot Y mod TS (zeros past K)
-    auto stage_row = [&](int Y) {
-        const int slot = ((Y % TS) + TS) % TS;
-        constexpr int CH = P::AROW / 8;  // 8-element chunks per LDS row
-        for (int e = tid; e < kMfmaM * CH; e += NT) {
-            const int m = e / CH, c = e % CH;
+    // zero LDS once: chunks past the slice width, rows of absent patterns and the
+    // reference copies outside the row stay zero for the whole kernel
+    for (int e = tid; e < (int)(P::lds_bytes / 16); e += NT) reinterpret_cast<uint4*>(lds_raw)[e] = uint4{0, 0, 0, 0};
+    __syncthreads();
+
+    // ---- staging through registers: global loads for step y + 1 are issued before the
+    // matrix-core work of step y and written to LDS at the start of step y + 1
+    V pfh[P::PF], pfl[P::PF];
+    auto load_prow = [&](int Y) {
+#pragma unroll
+        for (int i = 0; i < P::PF; ++i) {
+            const int e = tid + NT * i;
             V vh{}, vl{};
-            const uint32_t p = p0 + m;
-            if (p < npat && Y >= 0 && Y < SY && 8 * c < S) {
-                const size_t g = ((size_t)p * nb + roi) * slice_elems + (size_t)Y * S + 8 * c;
-                vh = *reinterpret_cast<const V*>(phi + g);
-                if constexpr (SPLIT) vl = *reinterpret_cast<const V*>(plo + g);
+            if (e < kMfmaM * CH && Y >= 0 && Y < SY) {
+                const int m = e / CH, c = e % CH;
+                if (p0 + m < npat) {
+                    const size_t g = ((size_t)(p0 + m) * nb + roi) * slice_elems + (size_t)Y * S + 8 * c;
+                    vh = *reinterpret_cast<const V*>(phi + g);
+                    if constexpr (SPLIT) vl = *reinterpret_cast<const V*>(plo + g);
+                }
             }
-            *reinterpret_cast<V*>(A + ((size_t)slot * kMfmaM + m) * P::AROW + 8 * c) = vh;
-            if constexpr (SPLIT)
-                *reinterpret_cast<V*>(A + ((size_t)(TS + slot) * kMfmaM + m) * P::AROW + 8 * c) = vl;
+            pfh[i] = vh;
+            pfl[i] = vl;
+        }
+    };
+    auto store_prow = [&](int Y) {
+        if (Y < 0 || Y >= SY) return;  // rows outside the slice are never read
+        const int slot = Y % TS;
+#pragma unroll
+        for (int i = 0; i < P::PF; ++i) {
+            const int e = tid + NT * i;
+            if (e < kMfmaM * CH) {
+                const int m = e / CH, c = e % CH;
+                const size_t o = ((size_t)slot * kMfmaM + m) * P::KMAX + 8 * (c ^ (m & 15));
+                *reinterpret_cast<V*>(A + o) = pfh[i];
+                if constexpr (SPLIT) *reinterpret_cast<V*>(A + P::ring + o) = pfl[i];
+            }
+        }
+    };
+    // reference row y as 8 shifted copies: Rs[rho][i] = R[y][i + rho - off] (0 outside
+    // the row), so the window R[y][q .. q+7] is Rs[rho][jp - rho ..] with jp = q + off,
+    // rho = jp & 7.  q spans [rx - 255, K16 - 8 + rx], hence jp in [1, K16 + 248].
+    HT rfh = HT(0.0f), rfl = HT(0.0f);
+    auto load_rrow = [&](int y) {
+        if (tid < S) {
+            rfh = refh[(size_t)y * S + tid];
+            if constexpr (SPLIT) rfl = refl[(size_t)y * S + tid];
+        }
+    };
+    auto store_rrow = [&]() {
+        if (tid < S) {
+#pragma unroll
+            for (int rho = 0; rho < 8; ++rho) {
+                const int i = tid - rho + off;
+                Rs[rho * P::RP + i] = rfh;
+                if constexpr (SPLIT) Rs[(8 + rho) * P::RP + i] = rfl;
+            }
         }
     };
 
     // y range: some y-shift of the tile overlaps reference row y
     const int y_lo = max(0, -(sy0 + TS - 1)), y_hi = min(SY, SY - sy0);
-    for (int Y = y_lo + sy0; Y < y_lo + sy0 + TS - 1; ++Y) stage_row(Y);
+    for (int Y = y_lo + sy0; Y < y_lo + sy0 + TS - 1; ++Y) {
+        load_prow(Y);
+        store_prow(Y);
+    }
+    if (y_lo < y_hi) {
+        load_prow(y_lo + sy0 + TS - 1);
+        load_rrow(y_lo);
+    }
+    const int m_a = lane % 32, half = lane / 32;
     for (int y = y_lo; y < y_hi; ++y) {
-        __syncthreads();  // previous step's reads of the ring slot and reference copies done
-        stage_row(y + sy0 + TS - 1);
-        // reference row y as 8 shifted copies: Rs[rho][i] = R[y][i + rho - off] (0 outside
-        // the row), so the window R[y][q .. q+7] is Rs[rho][jp - rho ..] with jp = q + off,
-        // rho = jp & 7.  q spans [rx - 255, K16 - 8 + rx], hence jp in [1, K16 + 248].
-        for (int e = tid; e < 8 * P::RP; e += NT) {
-            const int rho = e / P::RP, i = e % P::RP;
-            const int col = i + rho - off;
-            HT vh = HT(0.0f), vl = HT(0.0f);
-            if (col >= 0 && col < S) {
-                vh = refh[(size_t)y * S + col];
-                if constexpr (SPLIT) vl = refl[(size_t)y * S + col];
-            }
-            Rs[(size_t)rho * P::RP + i] = vh;
-            if constexpr (SPLIT) Rs[(size_t)(8 + rho) * P::RP + i] = vl;
-        }
+        store_prow(y + sy0 + TS - 1);
+        store_rrow();
         __syncthreads();
+        if (y + 1 < y_hi) {
+            load_prow(y + sy0 + TS);
+            load_rrow(y + 1);
+        }
+        for (int k0 = 0; k0 < K16; k0 += 16) {
+            // a tile is all zero unless some 0 <= k - sx < S with k in [k0, k0 + 16) and
+            // sx in [sx0, sx0 + 32)
+            bool nz[kMfmaTilesPerWave];
+            bool any = false;
 #pragma unroll
-        for (int u = 0; u < kMfmaTilesPerWave; ++u) {
-            const int sx0 = sxg0 + (wave * kMfmaTilesPerWave + u) * 32;
-            if (sx0 + rx >= (int)cross.x) continue;
-            for (int k0 = 0; k0 < K16; k0 += 16) {
-                // skip all-zero Toeplitz tiles: need some 0 <= k - sx < S with
-                // k in [k0, k0 + 16) and sx in [sx0, sx0 + 32)
-                if (k0 + 15 < sx0 || k0 - 31 - sx0 > S - 1) continue;
-                const int q = k0 + 8 * (lane / 32) - sx0 - (lane % 32);
-                const int jp = q + off;
+            for (int u = 0; u < kMfmaTilesPerWave; ++u) {
+                nz[u] = active[u] && !(k0 + 15 < sx0[u] || k0 - 31 - sx0[u] > S - 1);
+                any = any || nz[u];
+            }
+            if (!any) continue;
+            V bh[kMfmaTilesPerWave], bl[kMfmaTilesPerWave];
+#pragma unroll
+            for (int u = 0; u < kMfmaTilesPerWave; ++u) {
+                if (!nz[u]) continue;
+                const int jp = k0 + 8 * half - sx0[u] - m_a + off;
                 const int rho = jp & 7;
-                const V bh = *reinterpret_cast<const V*>(Rs + (size_t)rho * P::RP + (jp - rho));
-                V bl{};
-                if constexpr (SPLIT) bl = *reinterpret_cast<const V*>(Rs + (size_t)(8 + rho) * P::RP + (jp - rho));
+                bh[u] = *reinterpret_cast<const V*>(Rs + rho * P::RP + (jp - rho));
+                if constexpr (SPLIT) bl[u] = *reinterpret_cast<const V*>(Rs + (8 + rho) * P::RP + (jp - rho));
+            }
+            const int cs = 8 * ((k0 / 8 + half) ^ (m_a & 15));
 #pragma unroll
-                for (int t = 0; t < TS; ++t) {
-                    const int Y = y + sy0 + t;
-                    if (Y < 0 || Y >= SY) continue;
-                    const int slot = Y % TS;
-                    const HT* arow = A + ((size_t)slot * kMfmaM + (lane % 32)) * P::AROW + k0 + 8 * (lane / 32);
-                    const V ah = *reinterpret_cast<const V*>(arow);
-                    acc[t][u] = mfma_vec<HT>::mma(ah, bh, acc[t][u]);
+            for (int t = 0; t < TS; ++t) {
+                const int Y = y + sy0 + t;
+                if (Y < 0 || Y >= SY) continue;
+                const HT* arow = A + ((size_t)(Y % TS) * kMfmaM + m_a) * P::KMAX + cs;
+                const V ah = *reinterpret_cast<const V*>(arow);
+                V al{};
+                if constexpr (SPLIT) al = *reinterpret_cast<const V*>(arow + P::ring);
+#pragma unroll
+                for (int u = 0; u < kMfmaTilesPerWave; ++u) {
+                    if (!nz[u]) continue;
+                    acc[t][u] = mfma_vec<HT>::mma(ah, bh[u], acc[t][u]);
                     if constexpr (SPLIT) {
-                        const V al = *reinterpret_cast<const V*>(arow + (size_t)TS * kMfmaM * P::AROW);
-                        acc[t][u] = mfma_vec<HT>::mma(ah, bl, acc[t][u]);
-                        acc[t][u] = mfma_vec<HT>::mma(al, bh, acc[t][u]);
+                        acc[t][u] = mfma_vec<HT>::mma(ah, bl[u], acc[t][u]);
+                        acc[t][u] = mfma_vec<HT>::mma(al, bh[u], acc[t][u]);
                     }
                 }
             }
         }
+        __syncthreads();
     }
     // write the accumulators: D[8 (j/4) + 4 (l/32) + j%4][l%32]
 #pragma unroll
@@ -200,11 +272,11 @@ __global__ __launch_bounds__(kMfmaWaves * 64) void mfma_corr_kernel(const HT* __
         if (sy + ry >= (int)cross.y) continue;
 #pragma unroll
         for (int u = 0; u < kMfmaTilesPerWave; ++u) {
-            const int sx = sxg0 + (wave * kMfmaTilesPerWave + u) * 32 + (lane % 32);
+            const int sx = sx0[u] + m_a;
             if (sx + rx >= (int)cross.x) continue;
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
-                const uint32_t m = 8 * (j / 4) + 4 * (lane / 32) + (j % 4);
+                const uint32_t m = 8 * (j / 4) + 4 * half + (j % 4);
                 const uint32_t p = p0 + m;
                 if (p >= npat) continue;
                 corr[(((size_t)p * nb + roi) * cross.y + (sy + ry)) * cross.x + (sx + rx)] = acc[t][u][j];
@@ -214,7 +286,7 @@ __global__ __launch_bounds__(kMfmaWaves * 64) void mfma_corr_kernel(const HT* __
 }
 
 // --------------------------------------------------------------- launchers
-constexpr int kMfmaTS = 2;
+constexpr int kMfmaTS = 4;
 
 template <class HT, bool SPLIT>
 void launch_mfma_corr_t(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
@@ -224,10 +296,10 @@ void launch_mfma_corr_t(const HT* phi, const HT* plo, const HT* rhi, const HT* r
     static const hipError_t attr =
         hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P::lds_bytes);
     EMIDA_HIP(attr);
-    const uint64_t slabs = (uint64_t)nb * div_up(npat, (uint32_t)kMfmaM);
-    if (slabs > (1u << 22)) throw unsupported_error("matrix-core brute policy: too many ROI x pattern tiles");
-    dim3 grid((uint32_t)slabs, div_up(cross.y, (uint32_t)kMfmaTS));
-    kern<<<grid, kMfmaWaves * 64, P::lds_bytes, st>>>(phi, plo, rhi, rlo, corr, s, cross, nb, npat);
+    const uint64_t n_wg = (uint64_t)nb * div_up(npat, (uint32_t)kMfmaM) * div_up(cross.y, (uint32_t)kMfmaTS);
+    if (n_wg > (1u << 24)) throw unsupported_error("matrix-core brute policy: too many workgroups in one launch");
+    kern<<<(uint32_t)n_wg, kMfmaNT, P::lds_bytes, st>>>(phi, plo, rhi, rlo, corr, s, cross, nb, npat,
+                                                        (uint32_t)n_wg);
     EMIDA_LAUNCHED("mfma_corr_kernel");
 }
 

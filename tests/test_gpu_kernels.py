@@ -149,6 +149,12 @@ def test_cross_corr_64_fixture():  # test/cross_corr_test.cpp:205-217
     ((17, 23), (33, 45), 2, 1),
     ((64, 64), (9, 9), 2, 2),
     ((128, 128), (255, 255), 1, 1),
+    ((64, 48), (127, 95), 2, 17),    # two pattern tiles of the register-blocked kernel, the second partial
+    ((40, 24), (81, 49), 1, 3),      # the widest window the reference allows (2 size + 1)
+    ((24, 16), (32, 20), 2, 5),      # even window sizes
+    ((100, 30), (199, 59), 1, 2),    # 3 waves along x-shifts
+    ((136, 8), (15, 15), 1, 2),      # wider than 128: thread-per-shift kernel
+    ((128, 4), (257, 7), 1, 1),      # window wider than 256: thread-per-shift kernel
 ])
 def test_cross_corr_bitexact_vs_oracle(dt, size, res, ref_slices, batch):
     rng = np.random.default_rng(13)
@@ -157,6 +163,15 @@ def test_cross_corr_bitexact_vs_oracle(dt, size, res, ref_slices, batch):
     ref = rng.standard_normal(n * ref_slices).astype(dt)
     got = K.run_cross_corr(pics, ref, size, res, ref_slices, batch)
     np.testing.assert_array_equal(got, O.cross_corr(pics, ref, size, res, ref_slices, batch))
+
+
+def test_cross_corr_int32_wraps_like_oracle():
+    # the reference instantiates the brute kernel for int (test/cross_corr_test.cpp); sums wrap mod 2^32
+    rng = np.random.default_rng(2)
+    size, res = (48, 40), (95, 79)
+    pics = rng.integers(-40000, 40000, size=size[0] * size[1] * 3, dtype=np.int32)
+    ref = rng.integers(-40000, 40000, size=size[0] * size[1], dtype=np.int32)
+    np.testing.assert_array_equal(K.run_cross_corr(pics, ref, size, res, 1, 3), O.cross_corr(pics, ref, size, res, 1, 3))
 
 
 # ------------------------------------------------------------------ FFT correlation

@@ -10,7 +10,8 @@ from ._capi import (CROSS_POLICY_BRUTE, CROSS_POLICY_BRUTE_MFMA, CROSS_POLICY_FF
                     MFMA_BF16_SPLIT, MFMA_F16, MFMA_F16_SPLIT, EmidaError)
 from .gpu_offset import (DeviceBuffer, GpuOffset, PinnedBuffer, data_index_dtype, device_query,  # noqa: F401
                          get_offset, synchronize)
+from .serve import Server  # noqa: F401
 
 __all__ = ["CROSS_POLICY_BRUTE", "CROSS_POLICY_FFT", "CROSS_POLICY_BRUTE_MFMA", "MFMA_BF16", "MFMA_BF16_SPLIT",
            "MFMA_F16", "MFMA_F16_SPLIT", "EmidaError", "GpuOffset", "DeviceBuffer", "PinnedBuffer",
-           "data_index_dtype", "get_offset", "synchronize", "device_query", "LIB_PATH"]
+           "data_index_dtype", "get_offset", "synchronize", "device_query", "LIB_PATH", "Server"]

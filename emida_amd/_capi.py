@@ -42,6 +42,23 @@ class Size2(ctypes.Structure):
     _fields_ = [("x", ctypes.c_uint32), ("y", ctypes.c_uint32)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_uint64), ("max_batch", ctypes.c_uint32), ("streams", ctypes.c_uint32),
+                ("max_queued", ctypes.c_uint32), ("use_graphs", ctypes.c_int)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("submit_ns", ctypes.c_uint64), ("launch_ns", ctypes.c_uint64), ("done_ns", ctypes.c_uint64),
+                ("batch", ctypes.c_uint32), ("plan", ctypes.c_uint32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("launches", ctypes.c_uint64),
+                ("graph_launches", ctypes.c_uint64), ("graphs_captured", ctypes.c_uint64),
+                ("device_bytes_resident", ctypes.c_uint64), ("device_bytes_batches", ctypes.c_uint64),
+                ("device_bytes_peak_inflight", ctypes.c_uint64), ("slots", ctypes.c_uint32)]
+
+
 class OffsetInfo(ctypes.Structure):
     _fields_ = [("total_slices", ctypes.c_uint32), ("neigh_size", ctypes.c_uint32),
                 ("data_index_bytes", ctypes.c_uint32), ("cross_size", Size2), ("corr_buffer", Size2),
@@ -98,6 +115,14 @@ def lib():
             "emida_run_extract_neighbors": ([i32, i32, vp, sz, vp, vp, i32, S, u32, vp], i32),
             "emida_run_subpixel_max": ([i32, vp, i32, u32, vp, vp, vp], i32),
             "emida_lstsq_matrix": ([i32, dp], i32),
+            "emida_server_create": ([ctypes.POINTER(vp), ctypes.POINTER(ServerConfig)], i32),
+            "emida_server_destroy": ([vp], None),
+            "emida_server_add_plan": ([vp, i32, S, vp, ctypes.POINTER(S), u32, S, S, i32, i32, i32,
+                                       ctypes.POINTER(u32)], i32),
+            "emida_server_plan_info": ([vp, u32, ctypes.POINTER(OffsetInfo), ctypes.POINTER(u32)], i32),
+            "emida_server_submit": ([vp, u32, vp, dp, dp, ctypes.POINTER(u64)], i32),
+            "emida_server_wait": ([vp, u64, ctypes.POINTER(RequestInfo)], i32),
+            "emida_server_get_stats": ([vp, ctypes.POINTER(ServerStats)], i32),
             "emida_generate_hanning": ([i32, sz, vp], i32),
         }
         for name, (args, res) in sigs.items():

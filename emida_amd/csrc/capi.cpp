@@ -1,497 +1,5 @@
 // C ABI of the pattern-offset library (declared in include/emida_amd.h).
-//
-// emida_offset replaces the reference's gpu_offset<T, IN> (src/gpu_offset.hpp:48-326).
-// Differences of the host side, all deliberate:
-//  - three non-blocking streams (uploads, kernels, downloads) ordered by events;
-//    the reference launches its kernels on the legacy default stream;
-//  - an input buffer is not overwritten before the kernels that read it finished
-//    (the reference relies on queue capacities);
-//  - device results are double buffered so a download never races the next batch;
-//  - the subpixel fit runs on the GPU inside the peak/FFT kernels;
-//  - every buffer is released by emida_offset_destroy;
-//  - ROI bounds and sizes are validated up front instead of reading out of bounds.
-#include <algorithm>
-#include <atomic>
-#include <condition_variable>
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <memory>
-#include <mutex>
-#include <string>
-#include <vector>
-
-#include "emida_amd.h"
-#include "kernels.hpp"
-
-using namespace emida_amd;
-
-namespace {
-
-thread_local std::string g_last_error;
-
-struct state_error : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-int fail(int code, const std::string& msg) {
-    g_last_error = msg;
-    return code;
-}
-
-template <class F>
-int guarded(F&& f) {
-    try {
-        f();
-        return EMIDA_OK;
-    } catch (const arg_error& e) {
-        return fail(EMIDA_ERR_ARG, e.what());
-    } catch (const unsupported_error& e) {
-        return fail(EMIDA_ERR_UNSUPPORTED, e.what());
-    } catch (const state_error& e) {
-        return fail(EMIDA_ERR_STATE, e.what());
-    } catch (const hip_error& e) {
-        const std::string m = e.what();
-        if (m.find("out of memory") != std::string::npos || m.find("OutOfMemory") != std::string::npos)
-            return fail(EMIDA_ERR_NOMEM, m);
-        return fail(EMIDA_ERR_HIP, m);
-    } catch (const std::bad_alloc&) {
-        return fail(EMIDA_ERR_NOMEM, "host allocation failed");
-    } catch (const std::exception& e) {
-        return fail(EMIDA_ERR_HIP, e.what());
-    }
-}
-
-size_t type_size(emida_type t) {
-    switch (t) {
-        case EMIDA_F32: return 4;
-        case EMIDA_F64: return 8;
-        case EMIDA_I32: return 4;
-        case EMIDA_U16: return 2;
-    }
-    throw arg_error("unknown element type");
-}
-
-size2 S2(emida_size2 s) { return {s.x, s.y}; }
-
-// generate_hanning (src/host_helpers.hpp:29-37): double expression, then cast.
-template <class T>
-std::vector<T> hanning(size_t n) {
-    constexpr double PI = 3.14159265358979323846;
-    std::vector<T> w(n);
-    for (size_t i = 0; i < n; i++) w[i] = (T)(0.5 * (1 - cos(2 * PI * i / (n - 1))));
-    return w;
-}
-
-template <class F>
-void with_T(emida_type T, F&& f) {
-    if (T == EMIDA_F32) f(float{});
-    else if (T == EMIDA_F64) f(double{});
-    else throw arg_error("T must be F32 or F64");
-}
-
-// IN element type and its sum type (sums_trait, src/cufft_helpers.hpp:157-179).
-template <class F>
-void with_IN(emida_type IN, F&& f) {
-    if (IN == EMIDA_U16) f(uint16_t{}, uint32_t{});
-    else if (IN == EMIDA_F32) f(float{}, float{});
-    else if (IN == EMIDA_F64) f(double{}, double{});
-    else throw arg_error("IN must be U16, F32 or F64");
-}
-
-template <class X>
-struct tag {
-    using type = X;
-};
-
-// Half type of a matrix-core mode.
-template <class F>
-void with_HT(emida_mfma_mode mode, F&& f) {
-    if (mode == EMIDA_MFMA_BF16 || mode == EMIDA_MFMA_BF16_SPLIT) f(tag<__bf16>{});
-    else if (mode == EMIDA_MFMA_F16 || mode == EMIDA_MFMA_F16_SPLIT) f(tag<_Float16>{});
-    else throw arg_error("unknown matrix-core mode");
-}
-bool mfma_split(emida_mfma_mode mode) { return mode == EMIDA_MFMA_BF16_SPLIT || mode == EMIDA_MFMA_F16_SPLIT; }
-
-void* dmalloc(size_t bytes) {
-    void* p = nullptr;
-    if (bytes == 0) bytes = 16;
-    EMIDA_HIP(hipMalloc(&p, bytes));
-    return p;
-}
-
-}  // namespace
-
-struct emida_offset_s {
-    emida_type T, IN;
-    emida_cross_policy policy;
-    int fit;
-    size2 src, slice, cross, buf;
-    uint32_t nb, batch, total, n_in;
-    uint32_t side = 0;  // FFT slice side
-    std::vector<size2> begins;
-    size_t sum_bytes = 0;
-
-    int device = 0;
-    bool allocated = false;
-    hipStream_t in_stream = nullptr, compute = nullptr, out_stream = nullptr;
-    std::vector<void*> d_in;
-    std::vector<hipEvent_t> ev_up, ev_done;
-    std::atomic<uint32_t> rr{0};
-    std::mutex upload_mu;
-    std::condition_variable upload_cv;
-    std::vector<char> pending;  // buffer holds an upload no get_offset_core consumed yet
-
-    size2* d_begins = nullptr;
-    void* d_hx = nullptr;
-    void* d_hy = nullptr;
-    void* d_sums = nullptr;
-    void* d_ref = nullptr;     // brute: prepared reference slices; FFT: spectra
-    void* d_slices = nullptr;  // brute: prepared pattern slices
-    void* d_corr = nullptr;    // brute: correlation buffers + zero tail
-    size_t corr_elems = 0;
-    emida_mfma_mode mfma_mode = EMIDA_MFMA_BF16_SPLIT;
-    void* d_ref_lo = nullptr;  // matrix-core policy: lo halves of the reference slices
-    void* d_pic_hi = nullptr;  // matrix-core policy: prepared pattern slices, hi / lo halves
-    void* d_pic_lo = nullptr;
-
-    struct outset {
-        void* maxes = nullptr;
-        void* neigh = nullptr;
-        double* offsets = nullptr;
-        double* coefs = nullptr;
-        hipEvent_t copied = nullptr;
-        bool used = false;
-    } out[2];
-    uint32_t out_i = 0;
-
-    struct slot {
-        const void* maxes = nullptr;
-        const void* neigh = nullptr;
-        double* h_offsets = nullptr;
-        double* h_coefs = nullptr;
-        hipEvent_t done = nullptr;
-        uint64_t stamp = 0;
-    };
-    std::vector<slot> slots;
-    uint64_t stamp = 0;
-    std::mutex slot_mu;
-
-    bool timing = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> tev;
-    double t_ms = 0;
-    uint64_t t_launches = 0;
-
-    size_t device_bytes = 0;
-
-    size_t ti_bytes() const { return T == EMIDA_F32 ? 8 : 16; }
-    size_t t_bytes() const { return T == EMIDA_F32 ? 4 : 8; }
-    size_t in_bytes() const { return type_size(IN); }
-
-    void* alloc(size_t bytes) {
-        void* p = dmalloc(bytes);
-        device_bytes += bytes;
-        return p;
-    }
-
-    ~emida_offset_s() {
-        if (device >= 0) (void)hipSetDevice(device);
-        if (compute) (void)hipStreamSynchronize(compute);
-        if (in_stream) (void)hipStreamSynchronize(in_stream);
-        if (out_stream) (void)hipStreamSynchronize(out_stream);
-        for (void* p : d_in) (void)hipFree(p);
-        for (auto e : ev_up) (void)hipEventDestroy(e);
-        for (auto e : ev_done) (void)hipEventDestroy(e);
-        void* ptrs[] = {d_begins, d_hx, d_hy, d_sums, d_ref, d_slices, d_corr, d_ref_lo, d_pic_hi, d_pic_lo};
-        for (void* p : ptrs)
-            if (p) (void)hipFree(p);
-        for (auto& o : out) {
-            if (o.maxes) (void)hipFree(o.maxes);
-            if (o.neigh) (void)hipFree(o.neigh);
-            if (o.offsets) (void)hipFree(o.offsets);
-            if (o.coefs) (void)hipFree(o.coefs);
-            if (o.copied) (void)hipEventDestroy(o.copied);
-        }
-        for (auto& s : slots) {
-            if (s.h_offsets) (void)hipHostFree(s.h_offsets);
-            if (s.h_coefs) (void)hipHostFree(s.h_coefs);
-            if (s.done) (void)hipEventDestroy(s.done);
-        }
-        for (auto& p : tev) {
-            (void)hipEventDestroy(p.first);
-            (void)hipEventDestroy(p.second);
-        }
-        if (in_stream) (void)hipStreamDestroy(in_stream);
-        if (compute) (void)hipStreamDestroy(compute);
-        if (out_stream) (void)hipStreamDestroy(out_stream);
-    }
-
-    void require_allocated() const {
-        if (!allocated) throw state_error("allocate_memory must be called first");
-    }
-
-    // ---- reference preparation (src/gpu_offset.hpp:185-210)
-    void prepare_reference(const void* temp) {
-        void* d_temp = dmalloc((size_t)src.area() * in_bytes());
-        EMIDA_HIP(hipMemcpyAsync(d_temp, temp, (size_t)src.area() * in_bytes(), hipMemcpyHostToDevice, compute));
-        with_T(T, [&](auto t) {
-            using TT = decltype(t);
-            with_IN(IN, [&](auto in, auto sm) {
-                using IT = decltype(in);
-                using ST = decltype(sm);
-                const IT* pic = (const IT*)d_temp;
-                ST* sums = (ST*)d_sums;  // first nb entries are free before any batch runs
-                if (policy == EMIDA_CROSS_POLICY_BRUTE) {
-                    launch_sum<IT, ST>(pic, sums, d_begins, src, slice, nb, nb, compute);
-                    launch_prepare<IT, TT, ST>(pic, (TT*)d_ref, (const TT*)d_hx, (const TT*)d_hy, sums, d_begins,
-                                               src, slice, slice, nb, nb, compute);
-                } else if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA) {
-                    launch_sum<IT, ST>(pic, sums, d_begins, src, slice, nb, nb, compute);
-                    with_HT(mfma_mode, [&](auto ht) {
-                        using HT = typename decltype(ht)::type;
-                        launch_prepare_split<IT, ST, HT>(pic, (HT*)d_ref, (HT*)d_ref_lo, (const float*)d_hx,
-                                                         (const float*)d_hy, sums, d_begins, src, slice, nb, nb,
-                                                         compute);
-                    });
-                } else {
-                    if (!std::is_same<IT, uint16_t>::value)
-                        launch_sum<IT, ST>(pic, sums, d_begins, src, slice, nb, nb, compute);
-                    launch_fft_ref<TT, IT, ST>(pic, sums, (const TT*)d_hx, (const TT*)d_hy, d_begins, src, side, nb,
-                                               (cplx<TT>*)d_ref, compute);
-                }
-            });
-        });
-        EMIDA_HIP(hipStreamSynchronize(compute));
-        EMIDA_HIP(hipFree(d_temp));
-    }
-
-    void allocate(const void* temp) {
-        if (allocated) throw state_error("allocate_memory was already called");
-        EMIDA_HIP(hipGetDevice(&device));
-        EMIDA_HIP(hipStreamCreateWithFlags(&in_stream, hipStreamNonBlocking));
-        EMIDA_HIP(hipStreamCreateWithFlags(&compute, hipStreamNonBlocking));
-        EMIDA_HIP(hipStreamCreateWithFlags(&out_stream, hipStreamNonBlocking));
-        const size_t pic_bytes = (size_t)src.area() * batch * in_bytes();
-        for (uint32_t i = 0; i < n_in; ++i) {
-            d_in.push_back(alloc(pic_bytes));
-            hipEvent_t a, b;
-            EMIDA_HIP(hipEventCreateWithFlags(&a, hipEventDisableTiming));
-            EMIDA_HIP(hipEventCreateWithFlags(&b, hipEventDisableTiming));
-            ev_up.push_back(a);
-            ev_done.push_back(b);
-        }
-        pending.assign(n_in, 0);
-        d_begins = (size2*)alloc(begins.size() * sizeof(size2));
-        EMIDA_HIP(hipMemcpy(d_begins, begins.data(), begins.size() * sizeof(size2), hipMemcpyHostToDevice));
-        with_T(T, [&](auto t) {
-            using TT = decltype(t);
-            auto hx = hanning<TT>(slice.x), hy = hanning<TT>(slice.y);
-            d_hx = alloc(hx.size() * sizeof(TT));
-            d_hy = alloc(hy.size() * sizeof(TT));
-            EMIDA_HIP(hipMemcpy(d_hx, hx.data(), hx.size() * sizeof(TT), hipMemcpyHostToDevice));
-            EMIDA_HIP(hipMemcpy(d_hy, hy.data(), hy.size() * sizeof(TT), hipMemcpyHostToDevice));
-        });
-        d_sums = alloc(std::max(total, nb) * sum_bytes);
-        if (policy == EMIDA_CROSS_POLICY_BRUTE) {
-            d_ref = alloc((size_t)nb * slice.area() * t_bytes());
-            d_slices = alloc((size_t)total * slice.area() * t_bytes());
-            corr_elems = (size_t)total * cross.area() + cross.x + 1;
-            d_corr = alloc(corr_elems * t_bytes());
-            EMIDA_HIP(hipMemset(d_corr, 0, corr_elems * t_bytes()));
-        } else if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA) {
-            const size_t half = 2;  // bf16 / fp16
-            d_ref = alloc((size_t)nb * slice.area() * half);
-            d_pic_hi = alloc((size_t)total * slice.area() * half);
-            if (mfma_split(mfma_mode)) {
-                d_ref_lo = alloc((size_t)nb * slice.area() * half);
-                d_pic_lo = alloc((size_t)total * slice.area() * half);
-            }
-            corr_elems = (size_t)total * cross.area() + cross.x + 1;
-            d_corr = alloc(corr_elems * sizeof(float));
-            EMIDA_HIP(hipMemset(d_corr, 0, corr_elems * sizeof(float)));
-        } else {
-            const size_t spec = (size_t)(side + 1) * (2 * side) * 2 * t_bytes();
-            d_ref = alloc(spec * nb);
-        }
-        const size_t nk = (size_t)fit * fit;
-        for (auto& o : out) {
-            o.maxes = alloc(total * ti_bytes());
-            o.neigh = alloc(total * nk * t_bytes());
-            o.offsets = (double*)alloc(total * 2 * sizeof(double));
-            o.coefs = (double*)alloc(total * 6 * sizeof(double));
-            EMIDA_HIP(hipEventCreateWithFlags(&o.copied, hipEventDisableTiming));
-        }
-        (void)lstsq_device_table();
-        if (temp) prepare_reference(temp);
-        allocated = true;
-    }
-
-    // ---- one batch on device data (src/gpu_offset.hpp:234-278)
-    void run(const void* d_pics, uint32_t n_pat, void* maxes, void* neigh, double* offsets, double* coefs,
-             hipStream_t st) {
-        const uint32_t tn = nb * n_pat;
-        hipEvent_t t0 = nullptr, t1 = nullptr;
-        with_T(T, [&](auto t) {
-            using TT = decltype(t);
-            with_IN(IN, [&](auto in, auto sm) {
-                using IT = decltype(in);
-                using ST = decltype(sm);
-                const IT* pics = (const IT*)d_pics;
-                ST* sums = (ST*)d_sums;
-                if (policy == EMIDA_CROSS_POLICY_BRUTE) {
-                    launch_sum<IT, ST>(pics, sums, d_begins, src, slice, nb, tn, st);
-                    launch_prepare<IT, TT, ST>(pics, (TT*)d_slices, (const TT*)d_hx, (const TT*)d_hy, sums, d_begins,
-                                               src, slice, slice, nb, tn, st);
-                    if (timing) t0 = record_start(st);
-                    launch_cross_corr<TT>((const TT*)d_slices, (const TT*)d_ref, (TT*)d_corr, slice, cross, nb, n_pat,
-                                          st);
-                    if (timing) record_stop(st, t0);
-                    launch_peak<TT>((const TT*)d_corr, (size_t)tn * cross.area(), cross.area(), cross, POLICY_BRUTE,
-                                    fit, (data_index<TT>*)maxes, (TT*)neigh, offsets, coefs, tn, st);
-                } else if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA) {
-                    launch_sum<IT, ST>(pics, sums, d_begins, src, slice, nb, tn, st);
-                    with_HT(mfma_mode, [&](auto ht) {
-                        using HT = typename decltype(ht)::type;
-                        launch_prepare_split<IT, ST, HT>(pics, (HT*)d_pic_hi, (HT*)d_pic_lo, (const float*)d_hx,
-                                                         (const float*)d_hy, sums, d_begins, src, slice, nb, tn, st);
-                        if (timing) t0 = record_start(st);
-                        launch_mfma_corr<HT>((const HT*)d_pic_hi, (const HT*)d_pic_lo, (const HT*)d_ref,
-                                             (const HT*)d_ref_lo, (float*)d_corr, slice, cross, nb, n_pat, st);
-                        if (timing) record_stop(st, t0);
-                    });
-                    launch_peak<float>((const float*)d_corr, (size_t)tn * cross.area(), cross.area(), cross,
-                                       POLICY_BRUTE, fit, (data_index<float>*)maxes, (float*)neigh, offsets, coefs,
-                                       tn, st);
-                } else {
-                    if (!std::is_same<IT, uint16_t>::value)
-                        launch_sum<IT, ST>(pics, sums, d_begins, src, slice, nb, tn, st);
-                    if (timing) t0 = record_start(st);
-                    launch_fft_pipe<TT, IT, ST>(pics, sums, (const TT*)d_hx, (const TT*)d_hy, d_begins, src, side,
-                                                nb, tn, (const cplx<TT>*)d_ref, fit, (data_index<TT>*)maxes,
-                                                (TT*)neigh, offsets, coefs, nullptr, st);
-                    if (timing) record_stop(st, t0);
-                }
-            });
-        });
-        (void)t1;
-    }
-
-    hipEvent_t record_start(hipStream_t st) {
-        hipEvent_t a, b;
-        EMIDA_HIP(hipEventCreate(&a));
-        EMIDA_HIP(hipEventCreate(&b));
-        EMIDA_HIP(hipEventRecord(a, st));
-        tev.emplace_back(a, b);
-        return a;
-    }
-    void record_stop(hipStream_t st, hipEvent_t) { EMIDA_HIP(hipEventRecord(tev.back().second, st)); }
-
-    slot& slot_for(const void* maxes, const void* neigh) {
-        for (auto& s : slots)
-            if (s.maxes == maxes && s.neigh == neigh) return s;
-        if (slots.size() < 8) {
-            slots.emplace_back();
-            slot& s = slots.back();
-            EMIDA_HIP(hipHostMalloc((void**)&s.h_offsets, total * 2 * sizeof(double), hipHostMallocDefault));
-            EMIDA_HIP(hipHostMalloc((void**)&s.h_coefs, total * 6 * sizeof(double), hipHostMallocDefault));
-            EMIDA_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
-            s.maxes = maxes;
-            s.neigh = neigh;
-            return s;
-        }
-        slot* lru = &slots[0];
-        for (auto& s : slots)
-            if (s.stamp < lru->stamp) lru = &s;
-        EMIDA_HIP(hipEventSynchronize(lru->done));
-        lru->maxes = maxes;
-        lru->neigh = neigh;
-        return *lru;
-    }
-
-    void core(void* maxes_i, void* neighbors, uint32_t bi) {
-        require_allocated();
-        if (bi >= n_in) throw arg_error("buffer index out of range");
-        outset& o = out[out_i];
-        out_i ^= 1;
-        EMIDA_HIP(hipStreamWaitEvent(compute, ev_up[bi], 0));
-        if (o.used) EMIDA_HIP(hipStreamWaitEvent(compute, o.copied, 0));
-        run(d_in[bi], batch, o.maxes, o.neigh, o.offsets, o.coefs, compute);
-        EMIDA_HIP(hipEventRecord(ev_done[bi], compute));
-        {
-            std::lock_guard<std::mutex> lock(upload_mu);
-            pending[bi] = 0;
-        }
-        upload_cv.notify_all();
-        EMIDA_HIP(hipStreamWaitEvent(out_stream, ev_done[bi], 0));
-        const size_t nk = (size_t)fit * fit;
-        if (maxes_i) EMIDA_HIP(hipMemcpyAsync(maxes_i, o.maxes, total * ti_bytes(), hipMemcpyDeviceToHost, out_stream));
-        if (neighbors)
-            EMIDA_HIP(hipMemcpyAsync(neighbors, o.neigh, total * nk * t_bytes(), hipMemcpyDeviceToHost, out_stream));
-        std::lock_guard<std::mutex> lock(slot_mu);
-        slot& s = slot_for(maxes_i, neighbors);
-        s.stamp = ++stamp;
-        EMIDA_HIP(hipMemcpyAsync(s.h_offsets, o.offsets, total * 2 * sizeof(double), hipMemcpyDeviceToHost, out_stream));
-        EMIDA_HIP(hipMemcpyAsync(s.h_coefs, o.coefs, total * 6 * sizeof(double), hipMemcpyDeviceToHost, out_stream));
-        EMIDA_HIP(hipEventRecord(s.done, out_stream));
-        EMIDA_HIP(hipEventRecord(o.copied, out_stream));
-        o.used = true;
-    }
-
-    void finalize(const void* maxes_i, const void* neighbors, double* offsets, double* coefs) {
-        require_allocated();
-        const slot* found = nullptr;
-        slot copy;
-        {
-            std::lock_guard<std::mutex> lock(slot_mu);
-            for (auto& s : slots)
-                if (s.maxes == maxes_i && s.neigh == neighbors && s.stamp != 0) {
-                    copy = s;
-                    found = &copy;
-                    break;
-                }
-        }
-        if (found) {
-            // wait outside the lock so the next get_offset_core is not held up; the
-            // caller reuses these result buffers only after this returns
-            EMIDA_HIP(hipEventSynchronize(found->done));
-            if (offsets) std::memcpy(offsets, found->h_offsets, total * 2 * sizeof(double));
-            if (coefs) std::memcpy(coefs, found->h_coefs, total * 6 * sizeof(double));
-            return;
-        }
-        if (!maxes_i || !neighbors) throw arg_error("finalize: maxes_i and neighbors are required");
-        EMIDA_HIP(hipStreamSynchronize(out_stream));
-        // foreign buffers: fit them on the GPU
-        const size_t nk = (size_t)fit * fit;
-        void* dm = dmalloc(total * ti_bytes());
-        void* dn = dmalloc(total * nk * t_bytes());
-        double* doff = (double*)dmalloc(total * 2 * sizeof(double));
-        double* dco = (double*)dmalloc(total * 6 * sizeof(double));
-        try {
-            EMIDA_HIP(hipMemcpyAsync(dm, maxes_i, total * ti_bytes(), hipMemcpyHostToDevice, compute));
-            EMIDA_HIP(hipMemcpyAsync(dn, neighbors, total * nk * t_bytes(), hipMemcpyHostToDevice, compute));
-            const int pol = policy == EMIDA_CROSS_POLICY_FFT ? POLICY_FFT : POLICY_BRUTE;
-            with_T(T, [&](auto t) {
-                using TT = decltype(t);
-                launch_subpixel<TT>((const TT*)dn, fit, total, (const data_index<TT>*)dm, cross, pol, doff, dco,
-                                    compute);
-            });
-            if (offsets) EMIDA_HIP(hipMemcpyAsync(offsets, doff, total * 2 * sizeof(double), hipMemcpyDeviceToHost, compute));
-            if (coefs) EMIDA_HIP(hipMemcpyAsync(coefs, dco, total * 6 * sizeof(double), hipMemcpyDeviceToHost, compute));
-            EMIDA_HIP(hipStreamSynchronize(compute));
-        } catch (...) {
-            (void)hipStreamSynchronize(compute);
-            (void)hipFree(dm);
-            (void)hipFree(dn);
-            (void)hipFree(doff);
-            (void)hipFree(dco);
-            throw;
-        }
-        EMIDA_HIP(hipFree(dm));
-        EMIDA_HIP(hipFree(dn));
-        EMIDA_HIP(hipFree(doff));
-        EMIDA_HIP(hipFree(dco));
-    }
-};
+#include "offset.hpp"
 
 // ====================================================================== C ABI
 extern "C" {

@@ -243,6 +243,61 @@ int emida_lstsq_matrix(int s, double* out);
 /* generate_hanning<T>(size), src/host_helpers.hpp:29-37. */
 int emida_generate_hanning(emida_type T, size_t size, void* out);
 
+/* ======================================================================
+ * Request server (serving form).  No counterpart in the reference: its only driver,
+ * src/process_files.hpp:135-241, runs one fixed gpu_offset configuration over a file
+ * list.  Here single-pattern requests for several resident configurations ("plans")
+ * are batched continuously: a plan's queued requests launch together as soon as the
+ * plan has a free slot (a device buffer set for one batch); slots are created while
+ * their total size stays within hbm_budget; batches overlap on `streams` streams; a
+ * (slot, batch size) pipeline is captured in a hipGraph once and replayed.  Each
+ * request's results are the same as those of a lone request (the per-slice work does
+ * not depend on the other slices of a batch).  Patterns are uint16.
+ * ====================================================================== */
+typedef struct emida_server_s emida_server;
+
+typedef struct {
+    uint64_t hbm_budget; /* bytes of device memory for batch slots (inputs, working memory, results) */
+    uint32_t max_batch;  /* patterns per launch; lowered per plan so one slot fits in hbm_budget / 4 */
+    uint32_t streams;    /* device streams batches are spread over (1..16) */
+    uint32_t max_queued; /* pinned staging buffers: submitted requests not yet completed; submit blocks beyond */
+    int use_graphs;      /* replay hipGraph-captured pipelines (1) or launch kernels directly (0) */
+} emida_server_config;
+
+typedef struct {
+    uint64_t submit_ns, launch_ns, done_ns; /* steady clock of the library */
+    uint32_t batch;                         /* requests in the launch that carried this one */
+    uint32_t plan;
+} emida_request_info;
+
+typedef struct {
+    uint64_t submitted, completed, launches, graph_launches, graphs_captured;
+    uint64_t device_bytes_resident;       /* reference slices / spectra of all plans */
+    uint64_t device_bytes_batches;        /* sum of created slots */
+    uint64_t device_bytes_peak_inflight;  /* largest sum of slots in flight at once */
+    uint32_t slots;
+} emida_server_stats;
+
+int emida_server_create(emida_server** out, const emida_server_config* cfg);
+/* Completes every submitted request, then releases everything. */
+void emida_server_destroy(emida_server* s);
+/* A plan: the arguments of emida_offset_create (IN = U16) plus the reference pattern;
+ * its reference slices or spectra are prepared now and stay resident.  mfma_mode is
+ * used by EMIDA_CROSS_POLICY_BRUTE_MFMA only. */
+int emida_server_add_plan(emida_server* s, emida_type T, emida_size2 src_size, const uint16_t* ref,
+                          const emida_size2* begins, uint32_t n_begins, emida_size2 slice_size,
+                          emida_size2 cross_size, emida_cross_policy policy, emida_mfma_mode mfma_mode, int fitsize,
+                          uint32_t* plan_id);
+int emida_server_plan_info(emida_server* s, uint32_t plan_id, emida_offset_info* info, uint32_t* max_batch);
+/* Copies the pattern (src_size.x * src_size.y uint16) and queues it; offsets
+ * (double[2 * n_begins]) and coefs (double[6 * n_begins], may be null) are written
+ * before the request completes.  Blocks while max_queued requests are pending. */
+int emida_server_submit(emida_server* s, uint32_t plan_id, const uint16_t* pattern, double* offsets, double* coefs,
+                        uint64_t* request_id);
+/* Blocks until the request completed; info may be null.  A request is waited for once. */
+int emida_server_wait(emida_server* s, uint64_t request_id, emida_request_info* info);
+int emida_server_get_stats(emida_server* s, emida_server_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

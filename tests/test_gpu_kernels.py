@@ -165,13 +165,36 @@ def test_cross_corr_bitexact_vs_oracle(dt, size, res, ref_slices, batch):
     np.testing.assert_array_equal(got, O.cross_corr(pics, ref, size, res, ref_slices, batch))
 
 
+# Launches of at least 256 workgroups take the register-blocked kernel (16 patterns x
+# a band of shifts per workgroup); the shapes cover 1, 2, 3 and 4 waves along the
+# x-shifts, partial pattern tiles, even windows and the widest window allowed.
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("size,res,ref_slices,batch", [
+    ((32, 32), (63, 63), 70, 1),
+    ((64, 48), (127, 95), 12, 17),
+    ((128, 128), (255, 255), 4, 1),
+    ((40, 24), (81, 49), 37, 1),
+    ((100, 30), (199, 59), 18, 1),
+    ((24, 16), (32, 20), 130, 2),
+])
+def test_cross_corr_tiled_bitexact_vs_oracle(dt, size, res, ref_slices, batch):
+    rng = np.random.default_rng(17)
+    n = size[0] * size[1]
+    pics = rng.standard_normal(n * ref_slices * batch).astype(dt)
+    ref = rng.standard_normal(n * ref_slices).astype(dt)
+    got = K.run_cross_corr(pics, ref, size, res, ref_slices, batch)
+    np.testing.assert_array_equal(got, O.cross_corr(pics, ref, size, res, ref_slices, batch))
+
+
 def test_cross_corr_int32_wraps_like_oracle():
     # the reference instantiates the brute kernel for int (test/cross_corr_test.cpp); sums wrap mod 2^32
     rng = np.random.default_rng(2)
     size, res = (48, 40), (95, 79)
-    pics = rng.integers(-40000, 40000, size=size[0] * size[1] * 3, dtype=np.int32)
-    ref = rng.integers(-40000, 40000, size=size[0] * size[1], dtype=np.int32)
-    np.testing.assert_array_equal(K.run_cross_corr(pics, ref, size, res, 1, 3), O.cross_corr(pics, ref, size, res, 1, 3))
+    for nb, batch in ((1, 3), (16, 17)):  # thread-per-shift and register-blocked kernels
+        pics = rng.integers(-40000, 40000, size=size[0] * size[1] * nb * batch, dtype=np.int32)
+        ref = rng.integers(-40000, 40000, size=size[0] * size[1] * nb, dtype=np.int32)
+        np.testing.assert_array_equal(K.run_cross_corr(pics, ref, size, res, nb, batch),
+                                      O.cross_corr(pics, ref, size, res, nb, batch))
 
 
 # ------------------------------------------------------------------ FFT correlation

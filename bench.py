@@ -110,6 +110,159 @@ def copy_calibration(mb):
     print(json.dumps({"copy_calibration_bytes": n, "copies": 5}))
 
 
+def serve_plans(cfg, plain_fp64=False):
+    """(radius, k, T, policy, mfma mode) of every plan of the serving workload."""
+    import emida_amd
+
+    out = []
+    for r in cfg["radii"]:
+        for k in cfg["grids"]:
+            if plain_fp64:
+                out.append((r, k, np.float64, emida_amd.CROSS_POLICY_BRUTE, 0))
+            elif r == min(cfg["radii"]) or 2 * r > emida_amd.fft_max_slice(np.float32):
+                out.append((r, k, np.float32, emida_amd.CROSS_POLICY_BRUTE_MFMA, emida_amd.MFMA_BF16_SPLIT))
+            else:
+                out.append((r, k, np.float32, emida_amd.CROSS_POLICY_FFT, 0))
+    return out
+
+
+def serve_main(args, cfg):
+    """Serving workload: a request loop over resident plans (one per ROI radius and
+    grid), fed by a Poisson arrival process; reports p50/p99 per-pattern latency at
+    the offered load, then the saturation throughput of the same request mix, and the
+    same mix served by the plain FP64 brute-force form."""
+    import threading
+
+    import emida_amd
+    from emida_amd import synth
+
+    t_gen = time.perf_counter()
+    ps = synth.PatternSet(args.workload, cfg["size"], args.unique, t_range=cfg["t_range"])
+    t_gen = time.perf_counter() - t_gen
+    dev_name, cus = emida_amd.device_query()
+    print(f"# {dev_name}, {cus} CUs; generated {args.unique} distinct patterns in {t_gen:.1f}s", file=sys.stderr,
+          flush=True)
+    fit = cfg["fit"]
+
+    def make_server(plain_fp64):
+        srv = emida_amd.Server(hbm_budget=cfg["hbm_budget"], max_batch=args.serve_batch, streams=4,
+                               max_queued=4096, slots_per_plan=2)
+        plans = []
+        for r, k, T, pol, mode in serve_plans(cfg, plain_fp64):
+            begins = synth.grid_begins(cfg["size"], k, r)
+            pid = srv.add_plan(T, ps.ref, begins, (2 * r, 2 * r), (4 * r - 1, 4 * r - 1), pol, fit, mfma_mode=mode)
+            plans.append(dict(pid=pid, r=r, k=k, T=T, pol=pol, begins=begins))
+        return srv, plans
+
+    def run_load(srv, plans, n, rate, seed, submitters=1):
+        """n requests; Poisson arrivals at `rate` per second from one thread, or back to
+        back from `submitters` threads when rate is None."""
+        import queue
+
+        rng = np.random.default_rng(seed)
+        choice = rng.integers(0, len(plans), n)
+        pat = rng.integers(0, ps.pics.shape[0], n)
+        gaps = rng.exponential(1.0 / rate, n) if rate else np.zeros(n)
+        arrival = np.cumsum(gaps)
+        rids, outs, infos = [None] * n, [None] * n, [None] * n
+        done_q = queue.Queue()
+
+        def waiter():
+            for c in range(n):
+                i = done_q.get()
+                infos[i] = srv.wait(rids[i])
+                if c % 2000 == 1999:
+                    print(f"# {c + 1}/{n} requests done", file=sys.stderr, flush=True)
+
+        def submitter(idx, t0):
+            for i in idx:
+                if rate:
+                    while True:
+                        d = t0 + arrival[i] - time.perf_counter()
+                        if d <= 0:
+                            break
+                        time.sleep(d if d > 2e-4 else 0)
+                rid, offs, _ = srv.submit(plans[choice[i]]["pid"], ps.pics[pat[i]])
+                rids[i], outs[i] = rid, (choice[i], pat[i], offs)
+                done_q.put(i)
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        subs = [threading.Thread(target=submitter, args=(range(k, n, submitters), t0)) for k in range(submitters)]
+        for t in subs:
+            t.start()
+        for t in subs:
+            t.join()
+        th.join()
+        lat = np.array([inf["latency_ms"] for inf in infos])
+        t_first = min(inf["submit_ns"] for inf in infos)
+        t_last = max(inf["done_ns"] for inf in infos)
+        return dict(lat=lat, span_s=(t_last - t_first) * 1e-9, outs=outs,
+                    batch_mean=float(np.mean([inf["batch"] for inf in infos])))
+
+    srv, plans = make_server(False)
+    # warm-up: every plan captures its graphs for a few batch sizes
+    run_load(srv, plans, 40 * len(plans), None, 1, submitters=4)
+    n_load = int(cfg["rate"] * args.duration)
+    print(f"# offered load: {n_load} requests at {cfg['rate']:.0f}/s", file=sys.stderr, flush=True)
+    load = run_load(srv, plans, n_load, cfg["rate"], 2)
+    n_sat = args.saturation_requests
+    sat = run_load(srv, plans, n_sat, None, 3, submitters=4)
+    st = srv.stats()
+
+    # accuracy against the FP64 oracle on one pattern per plan (every 4th ROI)
+    from oracle import oracle as O
+    O.set_threads(min(16, len(os.sched_getaffinity(0))))
+    errs = {}
+    for p in plans:
+        rid, offs, coefs = srv.submit(p["pid"], ps.pics[0])
+        srv.wait(rid)
+        sub = p["begins"][::4]
+        eo, ec = O.get_offset(ps.pics[0], ps.ref, ps.size, sub, (2 * p["r"], 2 * p["r"]),
+                              (4 * p["r"] - 1, 4 * p["r"] - 1), 1, fit, O.POLICY_BRUTE, np.float64)
+        ok = well_posed(ec, fit)
+        e = np.abs(offs[::4] - eo).max(axis=1)
+        errs[f"r{p['r']}_k{p['k']}"] = float(e[ok].max()) if ok.any() else None
+    srv.close()
+
+    # the same request mix through the plain FP64 brute-force form (saturation)
+    srv64, plans64 = make_server(True)
+    run_load(srv64, plans64, len(plans64), None, 4)
+    sat64 = run_load(srv64, plans64, args.fp64_requests, None, 3, submitters=4)
+    srv64.close()
+
+    sat_rate = n_sat / sat["span_s"]
+    sat64_rate = args.fp64_requests / sat64["span_s"]
+    out = {"metric": METRIC, "value": round(sat_rate, 2), "unit": "patterns/s", "n_gpus": 1,
+           "steps": n_sat, "warmup": 40 * len(plans), "ms_per_step": round(1e3 / sat_rate, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 FFT (r32, r64) + bf16 split MFMA brute (r16), f32 accumulate", "data": "synthetic",
+           "config": {"workload": args.workload, "pattern": f"{cfg['size']}x{cfg['size']} uint16",
+                      "radii": list(cfg["radii"]), "roi_grids": [k * k for k in cfg["grids"]], "fitsize": fit,
+                      "plans": [f"r{p['r']}/{p['k'] * p['k']} ROIs/" +
+                                ("fft" if p["pol"] == emida_amd.CROSS_POLICY_FFT else "mfma-bf16-split")
+                                for p in plans],
+                      "offered_rate": cfg["rate"], "offered_seconds": args.duration,
+                      "hbm_budget": cfg["hbm_budget"], "max_batch": args.serve_batch, "streams": 4,
+                      "slots_per_plan": 2,
+                      "distinct_patterns": args.unique, "device": dev_name},
+           "offered_load": {"requests": n_load, "achieved_patterns_per_s": round(n_load / load["span_s"], 1),
+                            "p50_pattern_latency_ms": round(float(np.percentile(load["lat"], 50)), 4),
+                            "p99_pattern_latency_ms": round(float(np.percentile(load["lat"], 99)), 4),
+                            "max_pattern_latency_ms": round(float(load["lat"].max()), 4),
+                            "mean_batch": round(load["batch_mean"], 2)},
+           "saturation": {"requests": n_sat, "patterns_per_s": round(sat_rate, 1),
+                          "p50_pattern_latency_ms": round(float(np.percentile(sat["lat"], 50)), 4),
+                          "mean_batch": round(sat["batch_mean"], 2)},
+           "plain_fp64_brute_served_patterns_per_s": round(sat64_rate, 2),
+           "served_over_plain_fp64": round(sat_rate / sat64_rate, 2),
+           "max_err_px_vs_fp64_per_plan": errs,
+           "server_stats": {k: int(v) for k, v in st.items()}}
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +275,10 @@ def main():
     ap.add_argument("--copy-calibration-mb", type=float, default=0, help="only run a device copy of this size")
     ap.add_argument("--mfma", choices=sorted(MFMA_MODES), default=None,
                     help="brute workloads: matrix-core correlation in this mode instead of the plain form")
+    ap.add_argument("--duration", type=float, default=None, help="serving: seconds of offered load")
+    ap.add_argument("--serve-batch", type=int, default=32, help="serving: patterns per launch at most")
+    ap.add_argument("--saturation-requests", type=int, default=6000, help="serving: back-to-back requests")
+    ap.add_argument("--fp64-requests", type=int, default=90, help="serving: requests through the FP64 form")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -136,6 +293,11 @@ def main():
     from emida_amd import synth
 
     cfg = synth.CONFIGS[args.workload]
+    if cfg["policy"] == "serve":
+        if args.duration is None:
+            args.duration = cfg["duration"]
+        args.unique = min(args.unique, 32)
+        return serve_main(args, cfg)
     r = cfg["radius"]
     slice_size = (2 * r, 2 * r)
     cross = (4 * r - 1, 4 * r - 1)

@@ -44,7 +44,7 @@ class Size2(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_uint64), ("max_batch", ctypes.c_uint32), ("streams", ctypes.c_uint32),
-                ("max_queued", ctypes.c_uint32), ("use_graphs", ctypes.c_int)]
+                ("max_queued", ctypes.c_uint32), ("use_graphs", ctypes.c_int), ("slots_per_plan", ctypes.c_uint32)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -115,6 +115,7 @@ def lib():
             "emida_run_extract_neighbors": ([i32, i32, vp, sz, vp, vp, i32, S, u32, vp], i32),
             "emida_run_subpixel_max": ([i32, vp, i32, u32, vp, vp, vp], i32),
             "emida_lstsq_matrix": ([i32, dp], i32),
+            "emida_fft_max_slice": ([i32, ctypes.POINTER(u32)], i32),
             "emida_server_create": ([ctypes.POINTER(vp), ctypes.POINTER(ServerConfig)], i32),
             "emida_server_destroy": ([vp], None),
             "emida_server_add_plan": ([vp, i32, S, vp, ctypes.POINTER(S), u32, S, S, i32, i32, i32,

@@ -396,6 +396,14 @@ int emida_run_subpixel_max(emida_type T, const void* neighbors, int s, uint32_t 
     });
 }
 
+int emida_fft_max_slice(emida_type T, uint32_t* side) {
+    return guarded([&] {
+        if (!side) throw arg_error("null argument");
+        if (T != EMIDA_F32 && T != EMIDA_F64) throw arg_error("T must be F32 or F64");
+        *side = (uint32_t)fft_max_side(T == EMIDA_F64) / 2;
+    });
+}
+
 int emida_lstsq_matrix(int s, double* out) {
     return guarded([&] {
         if (!out) throw arg_error("null argument");

@@ -1,0 +1,196 @@
+// Building blocks of the LDS-resident FFT kernels (fft.hip, fft256.hip): complex
+// helpers, the in-register radix-2 DFT, the two-level transform over a lane group
+// and the reference's prepare expression.
+#pragma once
+
+#include <type_traits>
+
+#include "kernels.hpp"
+
+namespace emida_amd {
+
+// ------------------------------------------------------------------ complex helpers
+template <class T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) {
+    return {a.re + b.re, a.im + b.im};
+}
+template <class T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) {
+    return {a.re - b.re, a.im - b.im};
+}
+template <class T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> w) {
+    return {a.re * w.re - a.im * w.im, a.re * w.im + a.im * w.re};
+}
+template <class T>
+__device__ __forceinline__ cplx<T> cmul_conj(cplx<T> a, cplx<T> w) {  // a * conj(w)
+    return {a.re * w.re + a.im * w.im, a.im * w.re - a.re * w.im};
+}
+// The reference's product p * conj(r) (src/hadamard.cu:39-40), same expression.
+template <class T>
+__device__ __forceinline__ cplx<T> hadamard(cplx<T> p, cplx<T> r) {
+    return {p.re * r.re + p.im * r.im, -p.re * r.im + p.im * r.re};
+}
+
+// exp(-2 pi i j / 16), j < 8: correctly rounded cos/sin of multiples of pi/8.
+constexpr double kC1 = 0.9238795325112867, kS1 = 0.3826834323650898, kR2 = 0.7071067811865476;
+
+// v * W_16^j (conjugate twiddle when INV); j is a compile-time constant after unrolling.
+template <class T, bool INV>
+__device__ __forceinline__ cplx<T> twiddle16(cplx<T> v, int j) {
+    const T sg = INV ? T(-1) : T(1);
+    switch (j) {
+        case 0: return v;
+        case 4: return INV ? cplx<T>{-v.im, v.re} : cplx<T>{v.im, -v.re};
+        case 2: return cmul(v, cplx<T>{T(kR2), sg * T(-kR2)});
+        case 6: return cmul(v, cplx<T>{T(-kR2), sg * T(-kR2)});
+        case 1: return cmul(v, cplx<T>{T(kC1), sg * T(-kS1)});
+        case 3: return cmul(v, cplx<T>{T(kS1), sg * T(-kC1)});
+        case 5: return cmul(v, cplx<T>{T(-kS1), sg * T(-kC1)});
+        default: return cmul(v, cplx<T>{T(-kC1), sg * T(-kS1)});  // 7
+    }
+}
+
+__host__ __device__ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+__host__ __device__ constexpr int brev_c(int v, int bits) {
+    int r = 0;
+    for (int b = 0; b < bits; ++b) r |= ((v >> b) & 1) << (bits - 1 - b);
+    return r;
+}
+
+// In-register DFT of N <= 16 points, natural order in and out (radix-2
+// decimation in frequency; the final bit reversal only renames registers).
+template <class T, int N, bool INV, bool HALF_ZERO = false>
+__device__ __forceinline__ void dft_reg(cplx<T> (&x)[N]) {
+    static_assert(N >= 1 && N <= 16 && (N & (N - 1)) == 0, "N must be a power of two <= 16");
+    if constexpr (HALF_ZERO && N >= 2) {
+        // first pass with x[N/2..N-1] = 0: x[k] unchanged, x[k + N/2] = x[k] W^k
+#pragma unroll
+        for (int k = 0; k < N / 2; ++k) x[k + N / 2] = twiddle16<T, INV>(x[k], k * (16 / N));
+    }
+#pragma unroll
+    for (int len = HALF_ZERO ? N / 2 : N; len >= 2; len >>= 1) {
+        const int half = len / 2;
+#pragma unroll
+        for (int s0 = 0; s0 < N; s0 += len) {
+#pragma unroll
+            for (int k = 0; k < half; ++k) {
+                const cplx<T> a = x[s0 + k], b = x[s0 + k + half];
+                x[s0 + k] = cadd(a, b);
+                x[s0 + k + half] = twiddle16<T, INV>(csub(a, b), k * (16 / len));
+            }
+        }
+    }
+    cplx<T> y[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) y[i] = x[brev_c(i, ilog2(N))];
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = y[i];
+}
+
+// Intra-wave LDS hand-off between the lanes of one FFT group.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Forward transform: lane n2 holds x[n1] = input[L*n1 + n2]; on return lane j holds
+// X[(j + L*i) + R*k2] in x[i*L + k2].  scr: L*(R+1) elements of this group.
+// HALF_ZERO: the inputs x[R/2..R-1] are zero padding.
+template <class T, int R, int L, int M, bool HALF_ZERO>
+__device__ __forceinline__ void fft_fwd(cplx<T> (&x)[R], int lane, cplx<T>* scr, const cplx<T>* tw) {
+    constexpr int Q = R / L;
+    dft_reg<T, R, false, HALF_ZERO>(x);
+#pragma unroll
+    for (int k1 = 1; k1 < R; ++k1) x[k1] = cmul(x[k1], tw[lane * k1]);
+#pragma unroll
+    for (int k1 = 0; k1 < R; ++k1) scr[lane * (R + 1) + k1] = x[k1];
+    wave_sync();
+    cplx<T> y[Q][L];
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+#pragma unroll
+        for (int m = 0; m < L; ++m) y[i][m] = scr[m * (R + 1) + lane + L * i];
+    wave_sync();
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        dft_reg<T, L, false>(y[i]);
+#pragma unroll
+        for (int k2 = 0; k2 < L; ++k2) x[i * L + k2] = y[i][k2];
+    }
+}
+
+// Inverse (unnormalised) transform from the frequency layout to natural order:
+// on return lane n2 holds output[L*n1 + n2] in x[n1].
+template <class T, int R, int L, int M>
+__device__ __forceinline__ void fft_inv(cplx<T> (&x)[R], int lane, cplx<T>* scr, const cplx<T>* tw) {
+    constexpr int Q = R / L;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        cplx<T> y[L];
+#pragma unroll
+        for (int k2 = 0; k2 < L; ++k2) y[k2] = x[i * L + k2];
+        dft_reg<T, L, true>(y);
+        const int k1 = lane + L * i;
+#pragma unroll
+        for (int n2 = 0; n2 < L; ++n2) scr[n2 * (R + 1) + k1] = n2 == 0 ? y[0] : cmul_conj(y[n2], tw[n2 * k1]);
+    }
+    wave_sync();
+#pragma unroll
+    for (int k1 = 0; k1 < R; ++k1) x[k1] = scr[lane * (R + 1) + k1];
+    wave_sync();
+    dft_reg<T, R, true>(x);
+}
+
+// Prepared pixel (src/prepare_pics.cu:36-41), no contraction.
+template <class T, class IN>
+__device__ __forceinline__ T prep_px(IN px, T mean, T wx, T wy) {
+#pragma clang fp contract(off)
+    T v = (T)px;
+    v -= mean;
+    v = v * wx * wy;
+    return v;
+}
+
+template <class T>
+__device__ __forceinline__ T mean_of(T sum, uint32_t area) {
+#pragma clang fp contract(off)
+    return sum / (T)area;
+}
+
+// Workgroup -> slice (pattern * nb + roi).  Workgroups are dispatched round robin
+// over the 8 XCDs (b mod 8); XCD x takes a contiguous run of the ROI-major order
+// (roi * npat + pattern), so the patterns of a ROI run back to back on one XCD and
+// that ROI's reference spectrum stays in the XCD's L2.
+__device__ __forceinline__ uint32_t xcd_slice(uint32_t b, uint32_t total, uint32_t nb) {
+    const uint32_t o = xcd_remap(b, total);
+    const uint32_t npat = total / nb;
+    return (o % npat) * nb + o / npat;
+}
+
+template <class T, class IN, class SUM>
+struct fft_args {
+    const IN* pics;          // patterns (FROM_PATTERN)
+    const SUM* sums;         // per-slice sums (FROM_PATTERN, IN != uint16)
+    const T* hx;             // Hann windows (FROM_PATTERN)
+    const T* hy;
+    const size2* begins;     // ROI corners
+    size2 src;               // pattern size
+    uint32_t nb;             // ROIs per pattern
+    uint32_t total;          // slices in this launch
+    const T* prepared;       // prepared s x s slices (!FROM_PATTERN)
+    const cplx<T>* spectra;  // reference spectra (PIPE)
+    cplx<T>* spectra_out;    // reference spectra (!PIPE)
+    int fit;
+    const double* lsq;
+    data_index<T>* maxes;
+    T* neigh;
+    double* offsets;
+    double* coefs;
+    T* corr_out;             // optional (2s)^2 circular buffers, row-major
+    cplx<T>* work;           // two-kernel path: column-stage results of this launch's slices
+    uint32_t first;          // two-kernel path: first slice of this launch (a multiple of nb)
+};
+
+}  // namespace emida_amd

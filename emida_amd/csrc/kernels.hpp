@@ -42,9 +42,14 @@ template <class T>
 void launch_subpixel(const T* neigh, int s, uint32_t n, const data_index<T>* maxes, size2 cross, int policy,
                      double* offsets, double* coefs, hipStream_t st);
 
-// ---- fft.hip
-// Largest padded FFT side handled by the single-kernel FFT pipeline.
+// ---- fft.hip, fft256.hip
+// Largest padded FFT side: 128 for double (single-kernel pipeline), 256 for float
+// (M = 256 runs as two kernels with a work buffer, see fft256.hip).
 int fft_max_side(bool is_double);
+// M = 256: slices per launch chunk and the work buffer a launch of `total` slices needs.
+constexpr uint32_t kFft256Chunk = 4096;
+size_t fft256_chunk(uint32_t nb);
+size_t fft256_work_bytes(uint32_t nb, uint32_t total);
 // Reference spectra: prepared reference slices -> half spectra in the layout the
 // pipeline kernel reads ((M/2+1) columns x M bit-reversed rows per ROI).
 template <class T, class IN, class S>
@@ -52,11 +57,13 @@ void launch_fft_ref(const IN* pics, const S* sums, const T* hx, const T* hy, con
                     uint32_t side, uint32_t nb, cplx<T>* spectra, hipStream_t st);
 // Whole FFT pipeline of `total` slices (patterns x ROIs): prepare, R2C, multiply by
 // conj(reference spectrum), C2R, arg-max, gather, fit.  corr_out (optional)
-// receives the (2s)^2 circular correlation buffers.
+// receives the (2s)^2 circular correlation buffers; work (side 128 only) holds
+// fft256_work_bytes(nb, total) bytes.
 template <class T, class IN, class S>
 void launch_fft_pipe(const IN* pics, const S* sums, const T* hx, const T* hy, const size2* begins, size2 src,
                      uint32_t side, uint32_t nb, uint32_t total, const cplx<T>* spectra, int fit,
-                     data_index<T>* maxes, T* neigh, double* offsets, double* coefs, T* corr_out, hipStream_t st);
+                     data_index<T>* maxes, T* neigh, double* offsets, double* coefs, T* corr_out, void* work,
+                     hipStream_t st);
 // Prepared slices in, circular correlation buffers out (cross_corr_fft seam).
 template <class T>
 void launch_fft_corr_prepared(const T* pics, const T* ref, T* res, uint32_t side, uint32_t ref_slices,

@@ -207,6 +207,8 @@ struct emida_offset_s {
             parts[2] = al256(tn * cross.area() * sizeof(float));
             parts[3] = al256(tn * slice.area() * 2);
             if (mfma_split(mfma_mode)) parts[4] = parts[3];
+        } else if (side > 64) {
+            parts[2] = al256(fft256_work_bytes(nb, (uint32_t)tn));  // column-stage work buffer
         }
         return parts[0] + parts[1] + parts[2] + parts[3] + parts[4];
     }
@@ -356,6 +358,7 @@ struct emida_offset_s {
         } else {
             const size_t spec = (size_t)(side + 1) * (2 * side) * 2 * t_bytes();
             d_ref = alloc(spec * nb);
+            if (side > 64 && !resident_only) d_corr = alloc(fft256_work_bytes(nb, total));
         }
         const size_t nk = (size_t)fit * fit;
         for (auto& o : out) {
@@ -417,7 +420,7 @@ struct emida_offset_s {
                     if (timing) t0 = record_start(st);
                     launch_fft_pipe<TT, IT, ST>(pics, sums, (const TT*)d_hx, (const TT*)d_hy, d_begins, src, side,
                                                 nb, tn, (const cplx<TT>*)d_ref, fit, (data_index<TT>*)maxes,
-                                                (TT*)neigh, offsets, coefs, nullptr, st);
+                                                (TT*)neigh, offsets, coefs, nullptr, sc.corr, st);
                     if (timing) record_stop(st, t0);
                 }
             });

@@ -11,7 +11,9 @@
 // while the sum of slot sizes stays within the HBM budget, so admission is by
 // memory.  Batches go round-robin over several streams and overlap on the device.
 // Each (slot, batch size) pipeline - kernels plus the copies of the results into the
-// slot's pinned buffers - is captured once in a hipGraph and replayed.  One
+// slot's pinned buffers - is captured once in a hipGraph and replayed; batch sizes
+// are rounded up to a power of two for this, the extra slices computing on stale
+// slot contents whose results nobody reads.  One
 // completion thread per stream waits for its batches in order, hands results to the
 // requests and frees the slot and the staging buffers.
 #include <chrono>
@@ -56,7 +58,7 @@ struct emida_server_s {
         double* h_coefs = nullptr;
         hipEvent_t done = nullptr;
         size_t bytes = 0;
-        std::map<uint32_t, hipGraphExec_t> graphs;  // by batch size
+        std::map<uint32_t, hipGraphExec_t> graphs;  // by (power-of-two) batch size
     };
     struct plan_rt {
         std::unique_ptr<emida_offset_s> off;
@@ -65,6 +67,7 @@ struct emida_server_s {
         size_t slot_bytes = 0;
         std::deque<request*> queue;
         std::vector<slot*> free_slots;
+        uint32_t n_slots = 0;
     };
     struct launch_rec {
         slot* sl;
@@ -284,7 +287,9 @@ struct emida_server_s {
         for (size_t i = 0; i < plans.size(); ++i) {
             plan_rt& p = *plans[i];
             if (p.queue.empty()) continue;
-            const bool can = !p.free_slots.empty() || slot_bytes_total + p.slot_bytes <= cfg.hbm_budget;
+            const bool can = !p.free_slots.empty() ||
+                             ((cfg.slots_per_plan == 0 || p.n_slots < cfg.slots_per_plan) &&
+                              slot_bytes_total + p.slot_bytes <= cfg.hbm_budget);
             if (can && p.queue.front()->submit_ns < t) {
                 t = p.queue.front()->submit_ns;
                 best = (int)i;
@@ -306,12 +311,14 @@ struct emida_server_s {
                 p.free_slots.pop_back();
             } else {
                 slot_bytes_total += p.slot_bytes;
+                ++p.n_slots;
                 lock.unlock();
                 try {
                     sl = new_slot((uint32_t)pi);
                 } catch (...) {
                     lock.lock();
                     slot_bytes_total -= p.slot_bytes;
+                    --p.n_slots;
                     throw;
                 }
                 lock.lock();
@@ -338,7 +345,13 @@ struct emida_server_s {
     void launch(plan_rt& p, slot& sl, const std::vector<request*>& reqs, hipStream_t s) {
         emida_offset_s& o = *p.off;
         const uint32_t n = (uint32_t)reqs.size();
-        const size_t tn = (size_t)o.nb * n;
+        uint32_t nr = n;  // patterns the launch computes
+        if (cfg.use_graphs) {
+            nr = 1;
+            while (nr < n) nr *= 2;
+            nr = std::min(nr, p.max_batch);
+        }
+        const size_t tn = (size_t)o.nb * nr;
         const uint64_t t = now_ns();
         for (uint32_t i = 0; i < n; ++i) {
             reqs[i]->launch_ns = t;
@@ -347,12 +360,12 @@ struct emida_server_s {
                                      hipMemcpyHostToDevice, s));
         }
         auto body = [&](hipStream_t q) {
-            o.run(sl.pics, n, sl.maxes, sl.neigh, sl.offsets, sl.coefs, sl.scratch, q);
+            o.run(sl.pics, nr, sl.maxes, sl.neigh, sl.offsets, sl.coefs, sl.scratch, q);
             EMIDA_HIP(hipMemcpyAsync(sl.h_offsets, sl.offsets, tn * 2 * sizeof(double), hipMemcpyDeviceToHost, q));
             EMIDA_HIP(hipMemcpyAsync(sl.h_coefs, sl.coefs, tn * 6 * sizeof(double), hipMemcpyDeviceToHost, q));
         };
         if (cfg.use_graphs) {
-            auto it = sl.graphs.find(n);
+            auto it = sl.graphs.find(nr);
             if (it == sl.graphs.end()) {
                 hipGraph_t g = nullptr;
                 hipGraphExec_t ge = nullptr;
@@ -368,7 +381,7 @@ struct emida_server_s {
                 const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
                 EMIDA_HIP(e);
-                it = sl.graphs.emplace(n, ge).first;
+                it = sl.graphs.emplace(nr, ge).first;
                 std::lock_guard<std::mutex> lock(mu);
                 ++st.graphs_captured;
             }

@@ -242,6 +242,13 @@ def get_offset(pic, temp, size, cross_size, c_policy=CROSS_POLICY_BRUTE, T=np.fl
         g.close()
 
 
+def fft_max_slice(T=np.float32) -> int:
+    """Largest (power-of-two) slice side the FFT policy accepts for precision T."""
+    v = ctypes.c_uint32()
+    C.check(C.lib().emida_fft_max_slice(C.type_code(np.dtype(T)), ctypes.byref(v)))
+    return v.value
+
+
 def synchronize():
     C.check(C.lib().emida_device_synchronize())
 

@@ -7,6 +7,7 @@ Results per request are those of GpuOffset.get_offset on the same pattern.
 from __future__ import annotations
 
 import ctypes
+import threading
 
 import numpy as np
 
@@ -17,19 +18,25 @@ from .gpu_offset import CROSS_POLICY_BRUTE, MFMA_BF16_SPLIT
 class Server:
     """hbm_budget: bytes of device memory for batch slots; max_batch: patterns per
     launch; streams: concurrent device streams; max_queued: pending requests before
-    submit blocks; use_graphs: replay hipGraph-captured pipelines."""
+    submit blocks; use_graphs: replay hipGraph-captured pipelines; slots_per_plan:
+    batch slots one plan may hold (0: as many as the budget allows)."""
 
-    def __init__(self, hbm_budget=8 << 30, max_batch=16, streams=4, max_queued=256, use_graphs=True):
-        cfg = C.ServerConfig(int(hbm_budget), int(max_batch), int(streams), int(max_queued), 1 if use_graphs else 0)
+    def __init__(self, hbm_budget=8 << 30, max_batch=16, streams=4, max_queued=256, use_graphs=True,
+                 slots_per_plan=2):
+        cfg = C.ServerConfig(int(hbm_budget), int(max_batch), int(streams), int(max_queued), 1 if use_graphs else 0,
+                             int(slots_per_plan))
         h = ctypes.c_void_p()
         C.check(C.lib().emida_server_create(ctypes.byref(h), ctypes.byref(cfg)))
         self._h = h
         self._plans = []
+        self._pending = {}  # request id -> output arrays the library writes into
+        self._lock = threading.Lock()
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
-            C.lib().emida_server_destroy(self._h)
+            C.lib().emida_server_destroy(self._h)  # completes every pending request first
             self._h = ctypes.c_void_p()
+            self._pending.clear()
 
     def __del__(self):
         try:
@@ -68,7 +75,8 @@ class Server:
 
     def submit(self, pid, pattern, offsets=None, coefs=None):
         """Queues one pattern; returns (request id, offsets (n, 2), coefs (n, 6)).
-        The output arrays are filled when the request completes (see wait)."""
+        The output arrays are filled when the request completes (see wait); the
+        server keeps them alive until then."""
         n = self._plans[pid]["n_rois"]
         pattern = np.ascontiguousarray(pattern, dtype=np.uint16)
         if pattern.size != self._plans[pid]["src"][0] * self._plans[pid]["src"][1]:
@@ -79,14 +87,23 @@ class Server:
             coefs = np.empty((n, 6))
         rid = ctypes.c_uint64()
         dp = ctypes.POINTER(ctypes.c_double)
+        if offsets.shape != (n, 2) or offsets.dtype != np.float64 or not offsets.flags.c_contiguous:
+            raise ValueError("offsets must be a C-contiguous float64 (n_rois, 2) array")
+        if coefs.shape != (n, 6) or coefs.dtype != np.float64 or not coefs.flags.c_contiguous:
+            raise ValueError("coefs must be a C-contiguous float64 (n_rois, 6) array")
+        # the local references keep the arrays alive until they are registered
         C.check(C.lib().emida_server_submit(self._h, pid, C.ptr(pattern), offsets.ctypes.data_as(dp),
                                             coefs.ctypes.data_as(dp), ctypes.byref(rid)))
+        with self._lock:
+            self._pending[rid.value] = (offsets, coefs)
         return rid.value, offsets, coefs
 
     def wait(self, rid):
         """Blocks until the request completed; returns its timing record (ns)."""
         info = C.RequestInfo()
         C.check(C.lib().emida_server_wait(self._h, rid, ctypes.byref(info)))
+        with self._lock:
+            self._pending.pop(rid, None)
         return dict(submit_ns=info.submit_ns, launch_ns=info.launch_ns, done_ns=info.done_ns, batch=info.batch,
                     plan=info.plan, latency_ms=(info.done_ns - info.submit_ns) * 1e-6)
 

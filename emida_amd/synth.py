@@ -138,4 +138,8 @@ CONFIGS = {
                         t_range=3.0),
     "brute_873_r64": dict(size=873, patterns=128, k=10, radius=64, policy="brute", dtype="f64", fit=7, batch=128,
                           t_range=3.0),
+    # serving: Poisson arrivals of single patterns, each drawing a ROI radius and a
+    # k x k grid; FFT in float, matrix-core (split bf16) brute for the smallest radius
+    "serve_873": dict(size=873, rate=2000.0, duration=60.0, radii=(16, 32, 64), grids=(10, 15, 20), fit=5,
+                      hbm_budget=64 << 30, t_range=3.0, policy="serve", dtype="f32"),
 }

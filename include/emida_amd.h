@@ -237,6 +237,11 @@ int emida_run_extract_neighbors(emida_type T, emida_cross_policy pos_policy, con
 int emida_run_subpixel_max(emida_type T, const void* neighbors, int s, uint32_t n, double* offsets,
                            double* coefs, void* stream);
 
+/* Largest slice side the FFT policy accepts for element type T (slice sides are
+ * powers of two up to this).  No counterpart in the reference, where cuFFT plans any
+ * size. */
+int emida_fft_max_slice(emida_type T, uint32_t* side);
+
 /* get_lstsq_matrix(s), src/subpixel_max.cpp:5-20: (A^T A)^-1 A^T as 6 x s^2 doubles. */
 int emida_lstsq_matrix(int s, double* out);
 
@@ -252,7 +257,10 @@ int emida_generate_hanning(emida_type T, size_t size, void* out);
  * their total size stays within hbm_budget; batches overlap on `streams` streams; a
  * (slot, batch size) pipeline is captured in a hipGraph once and replayed.  Each
  * request's results are the same as those of a lone request (the per-slice work does
- * not depend on the other slices of a batch).  Patterns are uint16.
+ * not depend on the other slices of a batch).  Graphs exist per power-of-two batch
+ * size; a batch of n requests runs the graph of the next power of two, the extra
+ * slices computing on stale slot contents whose results are discarded.  Patterns are
+ * uint16.
  * ====================================================================== */
 typedef struct emida_server_s emida_server;
 
@@ -262,6 +270,7 @@ typedef struct {
     uint32_t streams;    /* device streams batches are spread over (1..16) */
     uint32_t max_queued; /* pinned staging buffers: submitted requests not yet completed; submit blocks beyond */
     int use_graphs;      /* replay hipGraph-captured pipelines (1) or launch kernels directly (0) */
+    uint32_t slots_per_plan; /* batch slots one plan may hold at once (0: as many as the budget allows) */
 } emida_server_config;
 
 typedef struct {

@@ -212,6 +212,19 @@ def test_cross_corr_fft_vs_oracle(s, dt, tol):
     assert (np.abs(got - exp) / scale).max() < tol * max(1, np.log2(2 * s))
 
 
+@pytest.mark.parametrize("ref_slices,batch", [(2, 3), (1, 1)])
+def test_cross_corr_fft_128_float_vs_oracle(ref_slices, batch):
+    # 128 x 128 slices (256-point transforms) run as two kernels with a work buffer
+    s = 128
+    rng = np.random.default_rng(99 + batch)
+    pics = rng.standard_normal(s * s * ref_slices * batch).astype(np.float32)
+    ref = rng.standard_normal(s * s * ref_slices).astype(np.float32)
+    got = K.run_cross_corr_fft(pics, ref, (s, s), ref_slices, batch).reshape(-1, 4 * s * s)
+    exp = O.fft_corr(pics, ref, (s, s), ref_slices, batch).reshape(-1, 4 * s * s).astype(np.float64)
+    scale = np.abs(exp).max(axis=1, keepdims=True)
+    assert (np.abs(got - exp) / scale).max() < 2e-6 * np.log2(2 * s)
+
+
 def test_cross_corr_fft_smoke_4x4():  # test/cross_corr_fft_test.cpp:8-42 (same inputs, with an assertion)
     pic = np.array([[1, 2, 3, 1], [4, 5, 6, 1], [7, 8, 9, 1], [7, 8, 9, 1]], float)
     ref = np.array([[6, 3, 2, 1], [12, 23, 3, 1], [1, 8, 9, 1], [7, 8, 9, 1]], float)

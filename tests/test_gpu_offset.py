@@ -140,7 +140,7 @@ def test_fft_double_vs_oracle(fit):
     assert np.abs(offs - eo)[ok].max() < 1e-9
 
 
-@pytest.mark.parametrize("s,fit", [(32, 5), (16, 3), (8, 3), (64, 7)])
+@pytest.mark.parametrize("s,fit", [(32, 5), (16, 3), (8, 3), (64, 7), (128, 7), (128, 9)])
 def test_fft_float_vs_oracle(s, fit):
     ps = _patterns(256, 2)
     begins = synth.grid_begins(256, 5, s // 2)
@@ -167,6 +167,19 @@ def test_fft_float_873_config_subset_vs_fp64_oracle():
 
 
 # ------------------------------------------------------------------ host API forms
+def test_fft_float_128_chunked_launch_vs_oracle():
+    # more slices than one launch chunk of the 256-point path (4096 slices): 2 patterns
+    # x 2209 ROIs of 128 x 128 on a 873 x 873 pattern, checked on every 97th ROI
+    ps = _patterns(873, 2, "fft128", t_range=3.0)
+    begins = synth.grid_begins(873, 47, 64)
+    offs, _, maxes, _ = _run(np.float32, FFT, ps, begins, (128, 128), (255, 255), 5, 2)
+    sub = begins[::97]
+    idx = np.concatenate([np.arange(0, len(begins), 97) + p * len(begins) for p in range(2)])
+    eo, _, em, _ = O.get_offset(ps.pics.ravel(), ps.ref, ps.size, sub, (128, 128), (255, 255), 2, 5, FFT,
+                                np.float64, return_intermediates=True)
+    assert np.abs(offs[idx] - eo).max() < 2e-3
+
+
 def test_pipeline_api_equals_get_offset():
     ps = _patterns(256, 4)
     begins = synth.grid_begins(256, 5, 16)

@@ -144,6 +144,19 @@ def test_small_budget_still_completes(workload):
         assert st["device_bytes_peak_inflight"] <= st["device_bytes_batches"]
 
 
+def test_dropped_output_arrays_stay_alive(workload):
+    # regression: a caller that drops the returned arrays must not leave the library
+    # writing into freed memory; the server holds them until the request is waited for
+    ps, plans = workload
+    srv, ids = _server(ps, plans[:1])
+    with srv:
+        rids = [srv.submit(ids[0], ps.pics[i % ps.pics.shape[0]])[0] for i in range(32)]
+        import gc
+        gc.collect()
+        for rid in rids:
+            srv.wait(rid)
+
+
 def test_errors():
     srv = emida_amd.Server(max_batch=2)
     with srv:

@@ -56,7 +56,9 @@ def test_create_valid_configurations_without_gpu():
     (dict(IN=np.int32), C.ERR_ARG),
     (dict(policy=emida_amd.CROSS_POLICY_FFT, slice_size=(48, 48)), C.ERR_UNSUPPORTED),
     (dict(policy=emida_amd.CROSS_POLICY_FFT, slice_size=(32, 16)), C.ERR_UNSUPPORTED),
-    (dict(policy=emida_amd.CROSS_POLICY_FFT, slice_size=(128, 128)), C.ERR_UNSUPPORTED),
+    (dict(policy=emida_amd.CROSS_POLICY_FFT, slice_size=(128, 128)), C.ERR_UNSUPPORTED),  # double: <= 64
+    (dict(policy=emida_amd.CROSS_POLICY_FFT, T=np.float32, slice_size=(256, 256), src_size=(300, 300)),
+     C.ERR_UNSUPPORTED),
     (dict(policy=emida_amd.CROSS_POLICY_BRUTE_MFMA), C.ERR_UNSUPPORTED),           # T must be float
     (dict(policy=emida_amd.CROSS_POLICY_BRUTE_MFMA, T=np.float32, slice_size=(36, 32)), C.ERR_UNSUPPORTED),
     (dict(policy=emida_amd.CROSS_POLICY_BRUTE_MFMA, T=np.float32, mfma_mode=7), C.ERR_ARG),
@@ -66,6 +68,12 @@ def test_create_rejects_invalid_arguments(kw, code):
     with pytest.raises(emida_amd.EmidaError) as e:
         _create(**kw)
     assert e.value.code == code
+
+
+def test_fft_limits():
+    assert emida_amd.fft_max_slice(np.float32) == 128 and emida_amd.fft_max_slice(np.float64) == 64
+    g = _create(policy=emida_amd.CROSS_POLICY_FFT, T=np.float32, slice_size=(128, 128), cross_size=(1, 1))
+    assert g.cross_size == (255, 255) and g.fft_size == 256
 
 
 def test_create_matrix_core_policy_without_gpu():

@@ -138,6 +138,9 @@ CONFIGS = {
                         t_range=3.0),
     "brute_873_r64": dict(size=873, patterns=128, k=10, radius=64, policy="brute", dtype="f64", fit=7, batch=128,
                           t_range=3.0),
+    # the precision-form shape through the FFT policy in float (256-point transforms)
+    "fft_f32_873_r64": dict(size=873, patterns=128, k=10, radius=64, policy="fft", dtype="f32", fit=7, batch=64,
+                            t_range=3.0),
     # serving: Poisson arrivals of single patterns, each drawing a ROI radius and a
     # k x k grid; FFT in float, matrix-core (split bf16) brute for the smallest radius
     "serve_873": dict(size=873, rate=2000.0, duration=60.0, radii=(16, 32, 64), grids=(10, 15, 20), fit=5,

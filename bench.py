@@ -55,11 +55,11 @@ def brute_flops_per_roi(s, cross):
     return 2.0 * ox * oy
 
 
-def read_traffic(workload):
-    """HBM bytes per launch of the dominant kernel from profiles/traffic_<workload>.txt
+def read_traffic(workload, mode=None):
+    """HBM bytes per launch of the dominant kernel from profiles/traffic_<workload>[_<mode>].txt
     (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, calibrated with a copy of known
     size).  None when no summary is committed."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.txt")
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}{'_' + mode if mode else ''}.txt")
     if not os.path.exists(path):
         return None
     text = open(path).read()
@@ -382,7 +382,7 @@ def main():
                       (FP64_PEAK_TFLOPS, "FP64 vector FMA"))
     achieved = flops / (k_avg_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "pipe": pipe, "kernel": kname, "achieved": round(achieved, 3), "peak": peak,
-                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": read_traffic(args.workload),
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": read_traffic(args.workload, args.mfma),
                 "flops_per_launch": flops, "avg_launch_ms": round(k_avg_ms, 4), "launches": k_n,
                 "algorithmic_bytes_per_launch": batch * pat_bytes + total * (g.data_index.itemsize + 16 + 48)
                 + g.neigh_size * np.dtype(T).itemsize}

@@ -234,6 +234,24 @@ __global__ __launch_bounds__(256, 2) void cross_corr_tiled_kernel(const T* __res
     }
 }
 
+// Workgroups of the register-blocked kernel for this launch, or 0 when the
+// thread-per-shift kernel runs instead (shape outside the tiled limits, or fewer
+// workgroups than CUs, where the finer grid of the thread-per-shift kernel is faster).
+template <class T>
+uint64_t tiled_workgroups(size2 size, size2 res_size, uint32_t ref_slices, uint32_t batch, tiled_shape& sh) {
+    if (!tiled_plan<T>(size, res_size, sh)) return 0;
+    const uint64_t n_wg =
+        (uint64_t)ref_slices * div_up(batch, (uint32_t)kTiledP) * div_up(res_size.y, (uint32_t)sh.rows);
+    return (n_wg >= 256 && n_wg <= (1u << 24)) ? n_wg : 0;
+}
+
+const char* cross_corr_kernel_name(bool is_double, size2 size, size2 res_size, uint32_t ref_slices, uint32_t batch) {
+    tiled_shape sh;
+    const uint64_t n = is_double ? tiled_workgroups<double>(size, res_size, ref_slices, batch, sh)
+                                 : tiled_workgroups<float>(size, res_size, ref_slices, batch, sh);
+    return n ? "cross_corr_tiled_kernel" : "cross_corr_kernel";
+}
+
 template <class T>
 void launch_cross_corr(const T* pics, const T* ref, T* res, size2 size, size2 res_size, uint32_t ref_slices,
                        uint32_t batch, hipStream_t st) {
@@ -241,23 +259,18 @@ void launch_cross_corr(const T* pics, const T* ref, T* res, size2 size, size2 re
         throw arg_error("cross_corr: res_size must be at most 2*size+1 per axis");
     const uint32_t total = ref_slices * batch;
     if (total == 0 || res_size.area() == 0) return;
-    if (total > 65535u) throw arg_error("cross_corr: more than 65535 slices in one launch");
     tiled_shape sh;
-    if (tiled_plan<T>(size, res_size, sh)) {
-        const uint64_t n_wg = (uint64_t)ref_slices * div_up(batch, (uint32_t)kTiledP) *
-                              div_up(res_size.y, (uint32_t)sh.rows);
-        // below one workgroup per CU the thread-per-shift kernel's finer grid is faster
-        if (n_wg >= 256 && n_wg <= (1u << 24)) {
-            auto kern = cross_corr_tiled_kernel<T>;
-            static const hipError_t attr =
-                hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
-            EMIDA_HIP(attr);
-            kern<<<(uint32_t)n_wg, 64 * sh.wx * sh.wy, sh.lds, st>>>(pics, ref, res, size, res_size, ref_slices,
-                                                                       batch, sh, (uint32_t)n_wg);
-            EMIDA_LAUNCHED("cross_corr_tiled_kernel");
-            return;
-        }
+    if (const uint64_t n_wg = tiled_workgroups<T>(size, res_size, ref_slices, batch, sh)) {
+        auto kern = cross_corr_tiled_kernel<T>;
+        static const hipError_t attr =
+            hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+        EMIDA_HIP(attr);
+        kern<<<(uint32_t)n_wg, 64 * sh.wx * sh.wy, sh.lds, st>>>(pics, ref, res, size, res_size, ref_slices, batch,
+                                                                   sh, (uint32_t)n_wg);
+        EMIDA_LAUNCHED("cross_corr_tiled_kernel");
+        return;
     }
+    if (total > 65535u) throw arg_error("cross_corr: more than 65535 slices in one launch");
     dim3 block(kBruteTX, kBruteTY);
     dim3 grid(div_up(res_size.x, (uint32_t)kBruteTX), div_up(res_size.y, (uint32_t)kBruteTY), total);
     const size_t lds = 2 * (size_t)size.area() * sizeof(T);

@@ -190,7 +190,9 @@ int emida_offset_get_timing(emida_offset* h, const char** kernel_name, double* t
         if (kernel_name)
             *kernel_name = h->policy == EMIDA_CROSS_POLICY_FFT
                                ? (h->side > 64 ? "fft256_cols_kernel+fft256_rows_kernel" : "fft_kernel")
-                               : (h->policy == EMIDA_CROSS_POLICY_BRUTE_MFMA ? "mfma_corr_kernel" : "cross_corr_kernel");
+                               : (h->policy == EMIDA_CROSS_POLICY_BRUTE_MFMA
+                                      ? "mfma_corr_kernel"
+                                      : cross_corr_kernel_name(h->T == EMIDA_F64, h->slice, h->cross, h->nb, h->batch));
         if (total_ms) *total_ms = h->t_ms;
         if (launches) *launches = h->t_launches;
     });

@@ -18,6 +18,8 @@ void launch_prepare(const IN* pics, T* slices, const T* hx, const T* hy, const S
 template <class T>
 void launch_cross_corr(const T* pics, const T* ref, T* res, size2 size, size2 res_size, uint32_t ref_slices,
                        uint32_t batch, hipStream_t st);
+// Name of the kernel launch_cross_corr picks for this shape (float or double).
+const char* cross_corr_kernel_name(bool is_double, size2 size, size2 res_size, uint32_t ref_slices, uint32_t batch);
 
 // ---- peak.hip
 // Offsets of the least-squares tables for fitsize 3, 5, 7, 9 in the device table.

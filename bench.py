@@ -120,7 +120,7 @@ def serve_plans(cfg, plain_fp64=False):
             if plain_fp64:
                 out.append((r, k, np.float64, emida_amd.CROSS_POLICY_BRUTE, 0))
             elif r == min(cfg["radii"]) or 2 * r > emida_amd.fft_max_slice(np.float32):
-                out.append((r, k, np.float32, emida_amd.CROSS_POLICY_BRUTE_MFMA, emida_amd.MFMA_BF16_SPLIT))
+                out.append((r, k, np.float32, emida_amd.CROSS_POLICY_BRUTE_MFMA, emida_amd.MFMA_F16_SPLIT))
             else:
                 out.append((r, k, np.float32, emida_amd.CROSS_POLICY_FFT, 0))
     return out
@@ -237,11 +237,11 @@ def serve_main(args, cfg):
     out = {"metric": METRIC, "value": round(sat_rate, 2), "unit": "patterns/s", "n_gpus": 1,
            "steps": n_sat, "warmup": 40 * len(plans), "ms_per_step": round(1e3 / sat_rate, 4),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f32 FFT (r32, r64) + bf16 split MFMA brute (r16), f32 accumulate", "data": "synthetic",
+           "dtype": "f32 FFT (r32, r64) + f16 split MFMA brute (r16), f32 accumulate", "data": "synthetic",
            "config": {"workload": args.workload, "pattern": f"{cfg['size']}x{cfg['size']} uint16",
                       "radii": list(cfg["radii"]), "roi_grids": [k * k for k in cfg["grids"]], "fitsize": fit,
                       "plans": [f"r{p['r']}/{p['k'] * p['k']} ROIs/" +
-                                ("fft" if p["pol"] == emida_amd.CROSS_POLICY_FFT else "mfma-bf16-split")
+                                ("fft" if p["pol"] == emida_amd.CROSS_POLICY_FFT else "mfma-f16-split")
                                 for p in plans],
                       "offered_rate": cfg["rate"], "offered_seconds": args.duration,
                       "hbm_budget": cfg["hbm_budget"], "max_batch": args.serve_batch, "streams": 4,
@@ -263,6 +263,117 @@ def serve_main(args, cfg):
     return 0
 
 
+def scan_main(args, cfg):
+    """Scan workload: patterns streamed back to back through the request loop, once
+    with the FFT policy in float and once with split-bf16 matrix-core brute
+    correlation; offsets of a subset compared with FP64 brute force on the GPU (the
+    FP64 brute kernels reproduce the CPU oracle bit for bit)."""
+    import queue
+    import threading
+
+    import emida_amd
+    from emida_amd import synth
+
+    t_gen = time.perf_counter()
+    sc = synth.ScanSet(args.workload, cfg["size"], args.unique, t_range=cfg["t_range"])
+    t_gen = time.perf_counter() - t_gen
+    dev_name, cus = emida_amd.device_query()
+    print(f"# {dev_name}, {cus} CUs; rendered {args.unique} raster points in {t_gen:.1f}s", file=sys.stderr,
+          flush=True)
+    r, fit = cfg["radius"], cfg["fit"]
+    s2, cross = (2 * r, 2 * r), (4 * r - 1, 4 * r - 1)
+    begins = synth.grid_begins(cfg["size"], cfg["k"], r)
+    n = args.scan_patterns or cfg["patterns"]
+
+    def stream(policy, mode):
+        srv = emida_amd.Server(hbm_budget=cfg["hbm_budget"], max_batch=args.serve_batch, streams=4,
+                               max_queued=512, slots_per_plan=4)
+        pid = srv.add_plan(np.float32, sc.ref, begins, s2, cross, policy, fit, mfma_mode=mode)
+        outs = {}
+        for i in range(min(64, n)):  # warm-up: graphs of every batch size
+            rid, o, _ = srv.submit(pid, sc.pics[i % args.unique])
+            srv.wait(rid)
+        done_q, infos = queue.Queue(), []
+
+        def waiter():
+            for _ in range(n):
+                rid, i, o = done_q.get()
+                infos.append(srv.wait(rid))
+                if i < args.unique:
+                    outs[i] = o.copy()
+                if len(infos) % 5000 == 0:
+                    print(f"# {len(infos)}/{n} patterns done", file=sys.stderr, flush=True)
+
+        def submitter(k, parts):
+            for i in range(k, n, parts):
+                rid, o, _ = srv.submit(pid, sc.pics[i % args.unique])
+                done_q.put((rid, i, o))
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        subs = [threading.Thread(target=submitter, args=(k, 4)) for k in range(4)]
+        for t in subs:
+            t.start()
+        for t in subs:
+            t.join()
+        th.join()
+        span = (max(i["done_ns"] for i in infos) - min(i["submit_ns"] for i in infos)) * 1e-9
+        lat = np.array([i["latency_ms"] for i in infos])
+        srv.close()
+        return n / span, lat, outs
+
+    fft_rate, fft_lat, fft_out = stream(emida_amd.CROSS_POLICY_FFT, 0)
+    mfma_rate, mfma_lat, mfma_out = stream(emida_amd.CROSS_POLICY_BRUTE_MFMA, emida_amd.MFMA_BF16_SPLIT)
+    f16_rate, f16_lat, f16_out = stream(emida_amd.CROSS_POLICY_BRUTE_MFMA, emida_amd.MFMA_F16_SPLIT)
+
+    # FP64 reference offsets of the rendered raster points (the subset), brute force on the GPU
+    sub = min(cfg["subset"], args.unique)
+    with emida_amd.GpuOffset(np.float64, np.uint16, sc.size, begins, s2, cross, 16, emida_amd.CROSS_POLICY_BRUTE,
+                             fit) as g:
+        g.allocate_memory(sc.ref)
+        ref_offs, ref_coefs = [], []
+        for b0 in range(0, sub, 16):
+            pics = np.zeros((16,) + sc.pics.shape[1:], np.uint16)
+            m = min(16, sub - b0)
+            pics[:m] = sc.pics[b0:b0 + m]
+            o, c = g.get_offset(pics)
+            ref_offs.append(o.reshape(16, -1, 2)[:m])
+            ref_coefs.append(c.reshape(16, -1, 6)[:m])
+    ref_offs = np.concatenate(ref_offs)
+    ref_coefs = np.concatenate(ref_coefs)
+    ok = well_posed(ref_coefs.reshape(-1, 6), fit).reshape(sub, -1)
+
+    def err(outs):
+        e = np.stack([np.abs(outs[i] - ref_offs[i]).max(axis=1) for i in range(sub)])
+        return {"max_px": float(e[ok].max()), "mean_px": float(e[ok].mean()),
+                "max_px_all_rois": float(np.nanmax(e))}
+
+    out = {"metric": METRIC, "value": round(fft_rate, 2), "unit": "patterns/s", "n_gpus": 1, "steps": n,
+           "warmup": min(64, n), "ms_per_step": round(1e3 / fft_rate, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32 FFT; bf16 / f16 split MFMA brute (f32 accumulate)",
+           "data": "synthetic",
+           "config": {"workload": args.workload, "pattern": f"{cfg['size']}x{cfg['size']} uint16",
+                      "patterns": n, "rois": len(begins), "roi": f"{2 * r}x{2 * r}", "fitsize": fit,
+                      "raster": "100 x 200 triangle, eps N(0, 2e-3) with 10-point correlation length",
+                      "distinct_patterns": args.unique, "fp64_subset": sub, "max_batch": args.serve_batch,
+                      "device": dev_name},
+           "fft_f32": {"patterns_per_s": round(fft_rate, 1), "roi_offsets_per_s": round(fft_rate * len(begins)),
+                       "p50_pattern_latency_ms": round(float(np.percentile(fft_lat, 50)), 3),
+                       "err_vs_fp64": err(fft_out)},
+           "mfma_bf16_split": {"patterns_per_s": round(mfma_rate, 1),
+                               "roi_offsets_per_s": round(mfma_rate * len(begins)),
+                               "p50_pattern_latency_ms": round(float(np.percentile(mfma_lat, 50)), 3),
+                               "err_vs_fp64": err(mfma_out)},
+           "mfma_f16_split": {"patterns_per_s": round(f16_rate, 1),
+                              "roi_offsets_per_s": round(f16_rate * len(begins)),
+                              "p50_pattern_latency_ms": round(float(np.percentile(f16_lat, 50)), 3),
+                              "err_vs_fp64": err(f16_out)},
+           "well_posed_roi_fraction": round(float(ok.mean()), 4),
+           "pcie_floor_patterns_per_s": round(50e9 / (cfg["size"] ** 2 * 2), 1)}
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,6 +390,7 @@ def main():
     ap.add_argument("--serve-batch", type=int, default=32, help="serving: patterns per launch at most")
     ap.add_argument("--saturation-requests", type=int, default=6000, help="serving: back-to-back requests")
     ap.add_argument("--fp64-requests", type=int, default=90, help="serving: requests through the FP64 form")
+    ap.add_argument("--scan-patterns", type=int, default=0, help="scan: patterns streamed (0: the workload's)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -293,6 +405,9 @@ def main():
     from emida_amd import synth
 
     cfg = synth.CONFIGS[args.workload]
+    if cfg["policy"] == "scan":
+        args.unique = min(args.unique, 64)
+        return scan_main(args, cfg)
     if cfg["policy"] == "serve":
         if args.duration is None:
             args.duration = cfg["duration"]

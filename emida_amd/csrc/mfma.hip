@@ -32,10 +32,15 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
+// Operands enter the half type scaled by `in` (a power of two, exact) and the
+// accumulators leave multiplied by `out` = 1 / in^2: fp16 keeps |v| < 65504 after
+// halving, which every prepared uint16 value satisfies (|px - mean| <= 65535, window
+// <= 1); bf16 has the fp32 exponent range and needs no scaling.
 template <class HT>
 struct mfma_vec;
 template <>
 struct mfma_vec<__bf16> {
+    static constexpr float in = 1.0f, out = 1.0f;
     using type = bf16x8;
     __device__ static f32x16 mma(bf16x8 a, bf16x8 b, f32x16 c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
@@ -43,6 +48,7 @@ struct mfma_vec<__bf16> {
 };
 template <>
 struct mfma_vec<_Float16> {
+    static constexpr float in = 0.5f, out = 4.0f;
     using type = f16x8;
     __device__ static f32x16 mma(f16x8 a, f16x8 b, f32x16 c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
@@ -68,6 +74,7 @@ __global__ __launch_bounds__(256) void prepare_split_kernel(const IN* __restrict
     float v = (float)pics[(sn / nb) * src.area() + (size_t)(b0.y + y) * src.x + b0.x + x];
     v -= (float)sums[sn] / (float)area;
     v = v * hx[x] * hy[y];
+    v *= mfma_vec<HT>::in;
     const HT h = (HT)v;
     hi[tid] = h;
     if (lo) lo[tid] = (HT)(v - (float)h);
@@ -279,7 +286,8 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
                 const uint32_t m = 8 * (j / 4) + 4 * half + (j % 4);
                 const uint32_t p = p0 + m;
                 if (p >= npat) continue;
-                corr[(((size_t)p * nb + roi) * cross.y + (sy + ry)) * cross.x + (sx + rx)] = acc[t][u][j];
+                corr[(((size_t)p * nb + roi) * cross.y + (sy + ry)) * cross.x + (sx + rx)] =
+                    acc[t][u][j] * mfma_vec<HT>::out;
             }
         }
     }
@@ -336,9 +344,10 @@ template <class HT>
 __global__ void split_kernel(const float* __restrict__ v, HT* __restrict__ hi, HT* __restrict__ lo, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const HT h = (HT)v[i];
+    const float w = v[i] * mfma_vec<HT>::in;
+    const HT h = (HT)w;
     hi[i] = h;
-    if (lo) lo[i] = (HT)(v[i] - (float)h);
+    if (lo) lo[i] = (HT)(w - (float)h);
 }
 
 template <class HT>

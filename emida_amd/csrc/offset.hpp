@@ -152,7 +152,7 @@ struct emida_offset_s {
     void* d_ref = nullptr;     // brute: prepared reference slices; FFT: spectra
     void* d_slices = nullptr;  // brute: prepared pattern slices
     void* d_corr = nullptr;    // brute: correlation buffers
-    emida_mfma_mode mfma_mode = EMIDA_MFMA_BF16_SPLIT;
+    emida_mfma_mode mfma_mode = EMIDA_MFMA_F16_SPLIT;
     void* d_ref_lo = nullptr;  // matrix-core policy: lo halves of the reference slices
     void* d_pic_hi = nullptr;  // matrix-core policy: prepared pattern slices, hi / lo halves
     void* d_pic_lo = nullptr;

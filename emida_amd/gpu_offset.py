@@ -103,7 +103,7 @@ class GpuOffset:
     """
 
     def __init__(self, T, IN, src_size, begins, slice_size, cross_size, batch_size, policy=CROSS_POLICY_BRUTE,
-                 s=3, in_buffers_size=1, mfma_mode=MFMA_BF16_SPLIT):
+                 s=3, in_buffers_size=1, mfma_mode=MFMA_F16_SPLIT):
         self.T = np.dtype(T)
         self.IN = np.dtype(IN)
         self.src_size = tuple(int(v) for v in src_size)

@@ -66,7 +66,7 @@ def run_cross_corr(pics, ref, size, res_size, ref_slices=1, batch_size=1):
     return out.download(pics.dtype)
 
 
-def run_cross_corr_mfma(pics, ref, size, res_size, ref_slices=1, batch_size=1, mode=C.MFMA_BF16_SPLIT):
+def run_cross_corr_mfma(pics, ref, size, res_size, ref_slices=1, batch_size=1, mode=C.MFMA_F16_SPLIT):
     """Matrix-core brute correlation of float32 prepared slices: same layout and
     meaning as run_cross_corr, products of bf16/fp16 halves accumulated in fp32
     (mode MFMA_*_SPLIT: hi*hi + hi*lo + lo*hi)."""

@@ -129,6 +129,50 @@ class PatternSet:
         return np.concatenate(out)
 
 
+def scan_raster(nx=100, ny=200, step=60.0):
+    """Triangle raster of a scan: x = step i + (step / 2)(j mod 2), y = floor(step j sqrt(0.75))."""
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny))
+    x = step * i + (step / 2) * (j % 2)
+    y = np.floor(step * j * math.sqrt(0.75))
+    return np.stack([x.ravel(), y.ravel()], axis=1), (ny, nx)
+
+
+def smooth_strain_field(shape, amplitude=2e-3, corr_length=10.0, seed=0):
+    """eps (2 x 2 per raster point) varying smoothly over the raster: white noise per
+    component, Gaussian-filtered with sigma = corr_length points, rescaled to the given
+    standard deviation.  Returns (ny * nx, 2, 2)."""
+    rng = np.random.default_rng(seed)
+    comps = []
+    for _ in range(4):
+        f = rng.standard_normal(shape)
+        f = _ndi.gaussian_filter(f, corr_length, mode="wrap") if _ndi is not None else f
+        comps.append(f / f.std() * amplitude)
+    return np.stack(comps, axis=-1).reshape(-1, 2, 2)
+
+
+class ScanSet:
+    """A scan: one reference and deformed patterns at raster points whose strain varies
+    smoothly over the raster.  Only the first `unique` raster points are rendered;
+    requests cycle over them."""
+
+    def __init__(self, name, size, unique, t_range=2.0, n_bands=24, seed=0):
+        self.name = name
+        self.size = (size, size) if np.isscalar(size) else tuple(size)
+        self.points, shape = scan_raster()
+        eps_field = smooth_strain_field(shape, seed=seed_of(name, seed, "eps"))
+        self.clean = band_reference(self.size, n_bands, seed_of(name, seed, "ref"))
+        self.ref = poisson_u16(self.clean, seed_of(name, seed, "ref-noise"))
+        w, h = self.size
+        self.pics = np.empty((unique, h, w), np.uint16)
+        self.eps, self.t = [], []
+        for i in range(unique):
+            rng = np.random.default_rng(seed_of(name, seed, "t", i))
+            t = rng.uniform(-t_range, t_range, 2)
+            self.pics[i] = poisson_u16(deform(self.clean, eps_field[i], t), seed_of(name, seed, "def-noise", i))
+            self.eps.append(eps_field[i])
+            self.t.append(t)
+
+
 # The five workloads of the benchmark configurations (pattern side, patterns,
 # ROI grid k x k, ROI radius, policy, precision, fitsize, batch).
 CONFIGS = {
@@ -145,4 +189,9 @@ CONFIGS = {
     # k x k grid; FFT in float, matrix-core (split bf16) brute for the smallest radius
     "serve_873": dict(size=873, rate=2000.0, duration=60.0, radii=(16, 32, 64), grids=(10, 15, 20), fit=5,
                       hbm_budget=64 << 30, t_range=3.0, policy="serve", dtype="f32"),
+    # a scan streamed through the request loop: 20,000 patterns of 1024 x 1024 on a
+    # triangle raster, 31 x 31 ROIs of 64 x 64, FFT float and split-bf16 matrix-core
+    # brute, fitsize 9, error vs FP64 on a subset
+    "scan_1024": dict(size=1024, patterns=20000, k=31, radius=32, fit=9, subset=500, t_range=2.0, policy="scan",
+                      dtype="f32", hbm_budget=64 << 30),
 }

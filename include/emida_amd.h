@@ -50,7 +50,9 @@ typedef enum {
 } emida_cross_policy;
 
 /* Operand precision of EMIDA_CROSS_POLICY_BRUTE_MFMA.  *_SPLIT modes store every
- * prepared value as hi + lo halves and accumulate hi*hi + hi*lo + lo*hi. */
+ * prepared value as hi + lo halves and accumulate hi*hi + hi*lo + lo*hi (relative
+ * representation error 2^-16 for bf16, 2^-22 for fp16).  fp16 operands are the
+ * prepared values halved, so any |value| < 131008 fits (all uint16 inputs). */
 typedef enum {
     EMIDA_MFMA_BF16 = 0,
     EMIDA_MFMA_BF16_SPLIT = 1,
@@ -148,7 +150,7 @@ int emida_offset_get_offset(emida_offset* h, const void* pic, double* offsets, d
 int emida_offset_process_device(emida_offset* h, const void* d_pics, uint32_t n_patterns, void* d_maxes,
                                 void* d_neighbors, double* d_offsets, double* d_coefs, void* stream);
 
-/* Operand precision of the BRUTE_MFMA policy (default EMIDA_MFMA_BF16_SPLIT); call
+/* Operand precision of the BRUTE_MFMA policy (default EMIDA_MFMA_F16_SPLIT); call
  * before emida_offset_allocate_memory.  No reference counterpart. */
 int emida_offset_set_mfma_mode(emida_offset* h, emida_mfma_mode mode);
 

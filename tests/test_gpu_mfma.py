@@ -25,8 +25,9 @@ pytestmark = pytest.mark.gpu
 MODES = {  # mode: (unit roundoff of the half type, split, subnormal half-spacing)
     emida_amd.MFMA_BF16: (2.0 ** -8, False, 0.0),
     emida_amd.MFMA_BF16_SPLIT: (2.0 ** -8, True, 0.0),
-    emida_amd.MFMA_F16: (2.0 ** -11, False, 2.0 ** -25),
-    emida_amd.MFMA_F16_SPLIT: (2.0 ** -11, True, 2.0 ** -25),
+    # fp16 rounds the halved value: its subnormal half-spacing 2^-25 is 2^-24 unscaled
+    emida_amd.MFMA_F16: (2.0 ** -11, False, 2.0 ** -24),
+    emida_amd.MFMA_F16_SPLIT: (2.0 ** -11, True, 2.0 ** -24),
 }
 SHAPES = [  # slice (x, y), cross (x, y), ROIs, patterns
     ((128, 128), (255, 255), 2, 3),

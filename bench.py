@@ -386,6 +386,8 @@ def main():
     ap.add_argument("--copy-calibration-mb", type=float, default=0, help="only run a device copy of this size")
     ap.add_argument("--mfma", choices=sorted(MFMA_MODES), default=None,
                     help="brute workloads: matrix-core correlation in this mode instead of the plain form")
+    ap.add_argument("--no-refine", action="store_true",
+                    help="with --mfma: use the float screen values directly (T = float) instead of the exact peak")
     ap.add_argument("--duration", type=float, default=None, help="serving: seconds of offered load")
     ap.add_argument("--serve-batch", type=int, default=32, help="serving: patterns per launch at most")
     ap.add_argument("--saturation-requests", type=int, default=6000, help="serving: back-to-back requests")
@@ -423,7 +425,9 @@ def main():
         if cfg["policy"] != "brute":
             print("bench.py: --mfma applies to brute workloads only", file=sys.stderr)
             return 2
-        T, policy = np.float32, emida_amd.CROSS_POLICY_BRUTE_MFMA
+        policy = emida_amd.CROSS_POLICY_BRUTE_MFMA
+        if args.no_refine:
+            T = np.float32
     batch, n_pat = cfg["batch"], cfg["patterns"]
     n_batches = n_pat // batch
 
@@ -437,7 +441,7 @@ def main():
           flush=True)
 
     g = emida_amd.GpuOffset(T, np.uint16, ps.size, begins, slice_size, cross, batch, policy, cfg["fit"],
-                            mfma_mode=mfma_code)
+                            mfma_mode=mfma_code, mfma_refine=not args.no_refine)
     g.allocate_memory(ps.ref)
     d_pics = emida_amd.DeviceBuffer.from_array(ps.pics)  # the whole job, resident in HBM
     pat_bytes = ps.size[0] * ps.size[1] * 2
@@ -505,11 +509,14 @@ def main():
     out = {"metric": METRIC, "value": round(pats_per_s, 2), "unit": "patterns/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": (f"{args.mfma} (f32 accumulate)" if args.mfma else "f32" if T == np.float32 else "f64"),
+           "dtype": ((f"{args.mfma} screen (f32 accumulate)" +
+                      ("" if args.no_refine else f", exact peak in {'f32' if T == np.float32 else 'f64'}"))
+                     if args.mfma else "f32" if T == np.float32 else "f64"),
            "data": "synthetic",
            "config": {"workload": args.workload, "pattern": f"{cfg['size']}x{cfg['size']} uint16",
                       "patterns": n_pat, "rois": n_roi, "roi": f"{2 * r}x{2 * r}",
-                      "policy": cfg["policy"] + (f"+mfma:{args.mfma}" if args.mfma else ""),
+                      "policy": cfg["policy"] + (f"+mfma:{args.mfma}" + ("" if args.no_refine else "+refine")
+                                                 if args.mfma else ""),
                       "fitsize": cfg["fit"], "batch": batch,
                       "distinct_patterns": args.unique, "device": dev_name},
            "roi_offsets_per_s": round(pats_per_s * n_roi, 1),

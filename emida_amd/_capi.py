@@ -93,6 +93,7 @@ def lib():
             "emida_offset_get_offset": ([vp, vp, dp, dp], i32),
             "emida_offset_process_device": ([vp, vp, u32, vp, vp, vp, vp, vp], i32),
             "emida_offset_set_mfma_mode": ([vp, i32], i32),
+            "emida_offset_set_mfma_refine": ([vp, i32], i32),
             "emida_offset_set_timing": ([vp, i32], i32),
             "emida_offset_get_timing": ([vp, ctypes.POINTER(ctypes.c_char_p), dp, ctypes.POINTER(u64)], i32),
             "emida_host_alloc": ([ctypes.POINTER(vp), sz], i32),

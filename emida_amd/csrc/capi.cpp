@@ -18,8 +18,6 @@ int emida_offset_create(emida_offset** out, emida_type T, emida_type IN, emida_s
         if (policy != EMIDA_CROSS_POLICY_BRUTE && policy != EMIDA_CROSS_POLICY_FFT &&
             policy != EMIDA_CROSS_POLICY_BRUTE_MFMA)
             throw arg_error("unknown policy");
-        if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA && T != EMIDA_F32)
-            throw unsupported_error("matrix-core brute policy: results are float (T must be F32)");
         if (fitsize != 3 && fitsize != 5 && fitsize != 7 && fitsize != 9)
             throw arg_error("fitsize must be 3, 5, 7 or 9");
         if (!begins || n_begins == 0) throw arg_error("at least one ROI is required");
@@ -164,6 +162,16 @@ int emida_offset_set_mfma_mode(emida_offset* h, emida_mfma_mode mode) {
         if (h->allocated) throw state_error("set_mfma_mode must precede allocate_memory");
         with_HT(mode, [](auto) {});
         h->mfma_mode = mode;
+    });
+}
+
+int emida_offset_set_mfma_refine(emida_offset* h, int enable) {
+    return guarded([&] {
+        if (!h) throw arg_error("null object");
+        if (h->allocated) throw state_error("set_mfma_refine must precede allocate_memory");
+        if (!enable && h->T != EMIDA_F32)
+            throw arg_error("the unrefined matrix-core screen gives float results: T must be F32");
+        h->mfma_refine = enable != 0;
     });
 }
 

@@ -44,6 +44,18 @@ template <class T>
 void launch_subpixel(const T* neigh, int s, uint32_t n, const data_index<T>* maxes, size2 cross, int policy,
                      double* offsets, double* coefs, hipStream_t st);
 
+// Precision form: exact peak from a float matrix-core screen of `total` slices and the
+// T-precision prepared pattern (p, [slice][s]) and reference (r, [roi][s]) slices;
+// outputs as launch_peak with the brute position policy.  The candidate bound is
+// tau = tol_rel ||p|| ||r|| + tol_abs (||p|| + ||r||) per slice.
+template <class T>
+void launch_refine(const float* screen, size_t corr_len, const T* p, const T* r, size2 s, size2 cross, uint32_t nb,
+                   double tol_rel, double tol_abs, int fit, data_index<T>* maxes, T* neigh, double* offsets,
+                   double* coefs, uint32_t total, hipStream_t st);
+// Screen error bound of launch_mfma_corr against the T-precision brute values, for
+// launch_refine (both values' rounding, counted twice): mode as emida_mfma_mode.
+void mfma_refine_bounds(int mode, size2 s, bool t_double, double& tol_rel, double& tol_abs);
+
 // ---- fft.hip, fft256.hip
 // Largest padded FFT side: 128 for double (single-kernel pipeline), 256 for float
 // (M = 256 runs as two kernels with a work buffer, see fft256.hip).

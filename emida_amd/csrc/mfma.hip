@@ -22,8 +22,10 @@
 // Matrix-core operand layout (v_mfma_f32_32x32x16_{bf16,f16}): lane l holds
 // A[l % 32][8 (l / 32) + t] and B[8 (l / 32) + t][l % 32] for t < 8; the 16
 // accumulators are D[8 (j / 4) + 4 (l / 32) + j % 4][l % 32], j < 16.
+#include <cmath>
 #include <type_traits>
 
+#include "emida_amd.h"
 #include "kernels.hpp"
 
 namespace emida_amd {
@@ -327,6 +329,32 @@ void launch_mfma_corr(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo
         launch_mfma_corr_t<HT, true>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st);
     else
         launch_mfma_corr_t<HT, false>(phi, nullptr, rhi, nullptr, corr, s, cross, nb, npat, st);
+}
+
+// |screen - T-precision brute value| <= tol_rel ||p|| ||r|| + tol_abs (||p|| + ||r||)
+// for one shift, counted twice (the screen maximum's and the candidate's errors):
+//  - each operand enters as a bf16/fp16 value (or hi + lo) of the float-prepared value:
+//    relative error e_op = 2u + u^2 (one half) or 3.1 u^2 (hi + lo, lo*lo dropped), plus
+//    2^-24 when T = double (the float rounding of the prepared value), plus for fp16 an
+//    absolute error of the subnormal spacing (2^-24 after undoing the halving);
+//  - products of halves are exact in fp32; their fp32 sum over n terms adds
+//    gamma_n = n 2^-24 / (1 - n 2^-24) of the sum of magnitudes;
+//  - the T-precision brute value itself is within gamma_n(u_T) of the exact sum;
+//  - sums of magnitudes over an overlap are bounded by ||p|| ||r|| (Cauchy-Schwarz) and
+//    by sqrt(n) ||p|| for the absolute terms.
+void mfma_refine_bounds(int mode, size2 s, bool t_double, double& tol_rel, double& tol_abs) {
+    const bool f16 = mode == EMIDA_MFMA_F16 || mode == EMIDA_MFMA_F16_SPLIT;
+    const bool split = mode == EMIDA_MFMA_BF16_SPLIT || mode == EMIDA_MFMA_F16_SPLIT;
+    const double u = f16 ? std::ldexp(1.0, -11) : std::ldexp(1.0, -8);
+    const double n = (double)s.x * s.y;
+    const double e_op = (split ? 3.1 * u * u : 2 * u + u * u) + (t_double ? 2.01 * std::ldexp(1.0, -24) : 0.0);
+    const double nt = n * (split ? 3 : 1), u32 = std::ldexp(1.0, -24);
+    const double g_acc = nt * u32 / (1 - nt * u32);
+    const double ut = t_double ? std::ldexp(1.0, -53) : u32;
+    const double g_t = n * ut / (1 - n * ut);
+    const double h = f16 ? 2.1 * std::ldexp(1.0, -24) : 0.0;
+    tol_rel = 2.02 * (e_op * (1 + g_acc) + g_acc * (1 + u) * (1 + u) + g_t);
+    tol_abs = 2.02 * h * std::sqrt(n) * (1 + g_acc) * (split ? 2 : 1);
 }
 
 template <class IN, class S, class HT>

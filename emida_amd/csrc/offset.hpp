@@ -153,7 +153,11 @@ struct emida_offset_s {
     void* d_slices = nullptr;  // brute: prepared pattern slices
     void* d_corr = nullptr;    // brute: correlation buffers
     emida_mfma_mode mfma_mode = EMIDA_MFMA_F16_SPLIT;
+    bool mfma_refine = true;   // matrix-core policy: exact peak from T-precision values
     void* d_ref_lo = nullptr;  // matrix-core policy: lo halves of the reference slices
+    void* d_ref_t = nullptr;   // matrix-core policy with refinement: T-precision reference slices
+    void* d_hxf = nullptr;     // matrix-core policy, T = double: float windows for the halves
+    void* d_hyf = nullptr;
     void* d_pic_hi = nullptr;  // matrix-core policy: prepared pattern slices, hi / lo halves
     void* d_pic_lo = nullptr;
 
@@ -204,6 +208,7 @@ struct emida_offset_s {
             parts[1] = al256(tn * slice.area() * t_bytes());
             parts[2] = al256(tn * cross.area() * t_bytes());
         } else if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA) {
+            if (mfma_refine) parts[1] = al256(tn * slice.area() * t_bytes());
             parts[2] = al256(tn * cross.area() * sizeof(float));
             parts[3] = al256(tn * slice.area() * 2);
             if (mfma_split(mfma_mode)) parts[4] = parts[3];
@@ -227,6 +232,8 @@ struct emida_offset_s {
         }
         return {ptr[0], ptr[1], ptr[2], ptr[3], ptr[4]};
     }
+    const void* float_hx() const { return d_hxf ? d_hxf : d_hx; }
+    const void* float_hy() const { return d_hyf ? d_hyf : d_hy; }
     scratch_t own_scratch() const { return {d_sums, d_slices, d_corr, d_pic_hi, d_pic_lo}; }
 
     size_t ti_bytes() const { return T == EMIDA_F32 ? 8 : 16; }
@@ -247,7 +254,8 @@ struct emida_offset_s {
         for (void* p : d_in) (void)hipFree(p);
         for (auto e : ev_up) (void)hipEventDestroy(e);
         for (auto e : ev_done) (void)hipEventDestroy(e);
-        void* ptrs[] = {d_begins, d_hx, d_hy, d_sums, d_ref, d_slices, d_corr, d_ref_lo, d_pic_hi, d_pic_lo};
+        void* ptrs[] = {d_begins, d_hx,     d_hy,     d_sums, d_ref,  d_slices, d_corr,
+                        d_ref_lo, d_pic_hi, d_pic_lo, d_ref_t, d_hxf,  d_hyf};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         for (auto& o : out) {
@@ -294,10 +302,13 @@ struct emida_offset_s {
                     launch_sum<IT, ST>(pic, sums, d_begins, src, slice, nb, nb, compute);
                     with_HT(mfma_mode, [&](auto ht) {
                         using HT = typename decltype(ht)::type;
-                        launch_prepare_split<IT, ST, HT>(pic, (HT*)d_ref, (HT*)d_ref_lo, (const float*)d_hx,
-                                                         (const float*)d_hy, sums, d_begins, src, slice, nb, nb,
+                        launch_prepare_split<IT, ST, HT>(pic, (HT*)d_ref, (HT*)d_ref_lo, (const float*)float_hx(),
+                                                         (const float*)float_hy(), sums, d_begins, src, slice, nb, nb,
                                                          compute);
                     });
+                    if (mfma_refine)
+                        launch_prepare<IT, TT, ST>(pic, (TT*)d_ref_t, (const TT*)d_hx, (const TT*)d_hy, sums,
+                                                   d_begins, src, slice, slice, nb, nb, compute);
                 } else {
                     if (!std::is_same<IT, uint16_t>::value)
                         launch_sum<IT, ST>(pic, sums, d_begins, src, slice, nb, nb, compute);
@@ -338,6 +349,13 @@ struct emida_offset_s {
             EMIDA_HIP(hipMemcpy(d_hx, hx.data(), hx.size() * sizeof(TT), hipMemcpyHostToDevice));
             EMIDA_HIP(hipMemcpy(d_hy, hy.data(), hy.size() * sizeof(TT), hipMemcpyHostToDevice));
         });
+        if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA && T == EMIDA_F64) {
+            auto hx = hanning<float>(slice.x), hy = hanning<float>(slice.y);
+            d_hxf = alloc(hx.size() * sizeof(float));
+            d_hyf = alloc(hy.size() * sizeof(float));
+            EMIDA_HIP(hipMemcpy(d_hxf, hx.data(), hx.size() * sizeof(float), hipMemcpyHostToDevice));
+            EMIDA_HIP(hipMemcpy(d_hyf, hy.data(), hy.size() * sizeof(float), hipMemcpyHostToDevice));
+        }
         d_sums = alloc((resident_only ? nb : std::max(total, nb)) * sum_bytes);
         const size_t scratch_total = resident_only ? 0 : total;
         if (policy == EMIDA_CROSS_POLICY_BRUTE) {
@@ -350,7 +368,9 @@ struct emida_offset_s {
             const size_t half = 2;  // bf16 / fp16
             d_ref = alloc((size_t)nb * slice.area() * half);
             if (mfma_split(mfma_mode)) d_ref_lo = alloc((size_t)nb * slice.area() * half);
+            if (mfma_refine) d_ref_t = alloc((size_t)nb * slice.area() * t_bytes());
             if (!resident_only) {
+                if (mfma_refine) d_slices = alloc(scratch_total * slice.area() * t_bytes());
                 d_pic_hi = alloc(scratch_total * slice.area() * half);
                 if (mfma_split(mfma_mode)) d_pic_lo = alloc(scratch_total * slice.area() * half);
                 d_corr = alloc(scratch_total * cross.area() * sizeof(float));
@@ -404,16 +424,28 @@ struct emida_offset_s {
                     launch_sum<IT, ST>(pics, sums, d_begins, src, slice, nb, tn, st);
                     with_HT(mfma_mode, [&](auto ht) {
                         using HT = typename decltype(ht)::type;
-                        launch_prepare_split<IT, ST, HT>(pics, (HT*)sc.pic_hi, (HT*)sc.pic_lo, (const float*)d_hx,
-                                                         (const float*)d_hy, sums, d_begins, src, slice, nb, tn, st);
+                        launch_prepare_split<IT, ST, HT>(pics, (HT*)sc.pic_hi, (HT*)sc.pic_lo,
+                                                         (const float*)float_hx(), (const float*)float_hy(), sums,
+                                                         d_begins, src, slice, nb, tn, st);
+                        if (mfma_refine)
+                            launch_prepare<IT, TT, ST>(pics, (TT*)sc.slices, (const TT*)d_hx, (const TT*)d_hy, sums,
+                                                       d_begins, src, slice, slice, nb, tn, st);
                         if (timing) t0 = record_start(st);
                         launch_mfma_corr<HT>((const HT*)sc.pic_hi, (const HT*)sc.pic_lo, (const HT*)d_ref,
                                              (const HT*)d_ref_lo, (float*)sc.corr, slice, cross, nb, n_pat, st);
                         if (timing) record_stop(st, t0);
                     });
-                    launch_peak<float>((const float*)sc.corr, (size_t)tn * cross.area(), cross.area(), cross,
-                                       POLICY_BRUTE, fit, (data_index<float>*)maxes, (float*)neigh, offsets, coefs,
-                                       tn, st);
+                    if (mfma_refine) {
+                        double tol_rel, tol_abs;
+                        mfma_refine_bounds(mfma_mode, slice, T == EMIDA_F64, tol_rel, tol_abs);
+                        launch_refine<TT>((const float*)sc.corr, (size_t)tn * cross.area(), (const TT*)sc.slices,
+                                          (const TT*)d_ref_t, slice, cross, nb, tol_rel, tol_abs, fit,
+                                          (data_index<TT>*)maxes, (TT*)neigh, offsets, coefs, tn, st);
+                    } else {
+                        launch_peak<float>((const float*)sc.corr, (size_t)tn * cross.area(), cross.area(), cross,
+                                           POLICY_BRUTE, fit, (data_index<float>*)maxes, (float*)neigh, offsets,
+                                           coefs, tn, st);
+                    }
                 } else {
                     if (!std::is_same<IT, uint16_t>::value)
                         launch_sum<IT, ST>(pics, sums, d_begins, src, slice, nb, tn, st);

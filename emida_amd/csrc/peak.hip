@@ -256,6 +256,274 @@ __global__ void subpixel_kernel(const T* __restrict__ neigh, int s, uint32_t n, 
         for (int row = 0; row < 6; ++row) coefs[6 * (size_t)i + row] = c[row];
 }
 
+// --------------------------------------------------------------- exact refinement
+// Precision form: the matrix-core screen (float) locates the peak; the values the
+// result depends on are then evaluated the way the brute kernels evaluate them (one
+// accumulator, y outer, x inner, fma; src/cross_corr.cu:36-61) from the T-precision
+// prepared slices, so they equal the brute policy's correlation values bit for bit.
+//   candidates: every shift whose screen value is within tau of the screen maximum,
+//     tau = tol_rel ||p|| ||r|| + tol_abs (||p|| + ||r||) bounding |screen - T value|
+//     twice over (caller's bound), so the brute policy's arg-max is among them;
+//   arg-max over the candidates' exact values, lowest index on ties;
+//   the F x F neighbourhood (identity position policy and clamp, spilling into the next
+//     slice as the brute path does) evaluated exactly; fit and finalize as peak_kernel.
+// A slice with more than kRefineCap candidates (a flat, noise-dominated screen) keeps
+// the screen's arg-max.
+constexpr int kRefineCap = 512;
+constexpr int kRefineNT = 128;
+
+template <class T>
+__device__ inline T exact_corr(const T* __restrict__ p, const T* __restrict__ r, size2 s, int sx, int sy) {
+    const int x_end = min((int)s.x - sx, (int)s.x), y_end = min((int)s.y - sy, (int)s.y);
+    const int x0 = max(-sx, 0);
+    T sum = 0;
+    for (int y = max(-sy, 0); y < y_end; ++y) {
+        const T* pr = p + (size_t)(y + sy) * s.x + sx;
+        const T* qr = r + (size_t)y * s.x;
+        for (int x = x0; x < x_end; ++x) sum = fma(pr[x], qr[x], sum);
+    }
+    return sum;
+}
+
+// Exact value of buffer element `g` of the launch (slice g / area, shift of g % area).
+template <class T>
+__device__ inline T exact_at(size_t g, size_t corr_len, const T* p, const T* r, size2 s, size2 cross, uint32_t nb) {
+    if (g >= corr_len) return T(0);
+    const uint32_t area = cross.area();
+    const uint32_t sn = (uint32_t)(g / area), pos = (uint32_t)(g % area);
+    const int sx = (int)(pos % cross.x) - (int)(cross.x - 1) / 2;
+    const int sy = (int)(pos / cross.x) - (int)(cross.y - 1) / 2;
+    const size_t a = (size_t)s.x * s.y;
+    return exact_corr<T>(p + (size_t)sn * a, r + (size_t)(sn % nb) * a, s, sx, sy);
+}
+
+// Exact values of a box of shifts of one slice, all threads of the workgroup
+// together: thread t < bw * bh takes position (x0 + t % bw, y0 + t / bw) of the cross
+// buffer (shift = position - (cross - 1) / 2).  Chunks of kBoxY reference rows and the
+// pattern rows they meet are staged in LDS (row stride s.x + 1); each thread
+// accumulates its own shift in the brute order (y outer, x inner, fma), loading 8
+// products' operands ahead of the 8 dependent fmas, so the values equal exact_corr's.
+// LDS: rrow kBoxY rows, prow kBoxY + kBoxRows - 1 rows.
+constexpr int kBoxRows = 12;
+constexpr int kBoxY = 4;
+
+template <class T, int NT>
+__device__ inline void eval_box(const T* __restrict__ p, const T* __restrict__ r, size2 s, size2 cross, int x0,
+                                int y0, int bw, int bh, T* rrow, T* prow, T* out) {
+    const int tid = threadIdx.x, S = (int)s.x, SY = (int)s.y;
+    const int PS = S + 1;  // LDS row stride: box rows of one wave fall on different banks
+    const int rx = (int)(cross.x - 1) / 2, ry = (int)(cross.y - 1) / 2;
+    const int sy_lo = y0 - ry, sy_hi = y0 + bh - 1 - ry;  // shift range of the box rows
+    const bool mine = tid < bw * bh;
+    const int sx = x0 + (mine ? tid % bw : 0) - rx, sy = y0 + (mine ? tid / bw : 0) - ry;
+    const int xa = max(-sx, 0), xb = min(S - sx, S);
+    T acc = 0;
+    const int ylo = max(0, -sy_hi), yhi = min(SY, SY - sy_lo);
+    for (int yc = ylo; yc < yhi; yc += kBoxY) {
+        const int ny = min(kBoxY, yhi - yc);
+        __syncthreads();
+        for (int e = tid; e < ny * S; e += NT) rrow[(e / S) * PS + e % S] = r[(size_t)(yc + e / S) * S + e % S];
+        for (int e = tid; e < (ny + bh - 1) * S; e += NT) {
+            const int Y = yc + sy_lo + e / S;
+            prow[(e / S) * PS + e % S] = (Y >= 0 && Y < SY) ? p[(size_t)Y * S + e % S] : T(0);
+        }
+        __syncthreads();
+        if (!mine) continue;
+        for (int k = 0; k < ny; ++k) {
+            const int y = yc + k;
+            if (y + sy < 0 || y + sy >= SY) continue;
+            const T* pr = prow + (size_t)(k + sy - sy_lo) * PS + sx;
+            const T* qr = rrow + (size_t)k * PS;
+            int x = xa;
+            for (; x + 8 <= xb; x += 8) {
+                T u[8], v[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    u[i] = pr[x + i];
+                    v[i] = qr[x + i];
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) acc = fma(u[i], v[i], acc);
+            }
+            for (; x < xb; ++x) acc = fma(pr[x], qr[x], acc);
+        }
+    }
+    if (mine) out[tid] = acc;
+    __syncthreads();
+}
+
+
+
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ screen, size_t corr_len,
+                                                    const T* __restrict__ p, const T* __restrict__ r, size2 s,
+                                                    size2 cross, uint32_t nb, double tol_rel, double tol_abs,
+                                                    int fit, const double* __restrict__ lsq,
+                                                    data_index<T>* __restrict__ maxes, T* __restrict__ neigh,
+                                                    double* __restrict__ offsets, double* __restrict__ coefs,
+                                                    uint32_t total) {
+    __shared__ float svf[NT / kWave];
+    __shared__ uint32_t si[NT / kWave];
+    __shared__ double sd[NT / kWave];
+    __shared__ T sv[NT / kWave];
+    __shared__ uint32_t cand[kRefineCap];
+    __shared__ uint32_t ncand;
+    __shared__ T nbv[81];
+    __shared__ double cf[6];
+    __shared__ T boxv[NT];
+    __shared__ int bb[4];
+    extern __shared__ __align__(16) unsigned char refine_lds[];
+    T* rrow = reinterpret_cast<T*>(refine_lds);  // kBoxY rows of s.x + 1
+    T* prow = rrow + kBoxY * (s.x + 1);           // kBoxY + kBoxRows - 1 rows of s.x + 1
+    const uint32_t sn = blockIdx.x;
+    if (sn >= total) return;
+    const int tid = threadIdx.x;
+    const uint32_t area = cross.area();
+    const size_t a = (size_t)s.x * s.y;
+    const float* sc = screen + (size_t)sn * area;
+    const T* ps = p + (size_t)sn * a;
+    const T* rs = r + (size_t)(sn % nb) * a;
+
+    // screen maximum (NaN ignored) and its lowest index; 8 loads in flight per thread
+    constexpr int U = 8;
+    float m = -__builtin_huge_valf();
+    uint32_t mi = 0xffffffffu;
+    for (uint32_t i0 = tid; i0 < area; i0 += NT * U) {
+        float v[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) v[k] = i0 + k * NT < area ? sc[i0 + k * NT] : -__builtin_huge_valf();
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+            if (v[k] > m) {
+                m = v[k];
+                mi = i0 + k * NT;
+            }
+    }
+    block_argmax<float, NT>(m, mi, svf, si);
+
+    // ||p||, ||r|| in double
+    double pp = 0, rr = 0;
+    for (size_t i0 = tid; i0 < a; i0 += NT * U) {
+        double u[U], w[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            u[k] = i0 + k * NT < a ? (double)ps[i0 + k * NT] : 0.0;
+            w[k] = i0 + k * NT < a ? (double)rs[i0 + k * NT] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            pp += u[k] * u[k];
+            rr += w[k] * w[k];
+        }
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        pp += __shfl_xor(pp, off);
+        rr += __shfl_xor(rr, off);
+    }
+    if (tid % kWave == 0) sd[tid / kWave] = pp;
+    __syncthreads();
+    pp = 0;
+    for (int k = 0; k < NT / kWave; ++k) pp += sd[k];
+    __syncthreads();
+    if (tid % kWave == 0) sd[tid / kWave] = rr;
+    __syncthreads();
+    rr = 0;
+    for (int k = 0; k < NT / kWave; ++k) rr += sd[k];
+    const double np = sqrt(pp), nr = sqrt(rr);
+    const double tau = tol_rel * np * nr + tol_abs * (np + nr);
+
+    // candidates
+    if (tid == 0) ncand = 0;
+    __syncthreads();
+    const double thr = (double)m - tau;
+    for (uint32_t i0 = tid; i0 < area; i0 += NT * U) {
+        float v[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) v[k] = i0 + k * NT < area ? sc[i0 + k * NT] : -__builtin_huge_valf();
+#pragma unroll
+        for (int k = 0; k < U; ++k)
+            if ((double)v[k] >= thr) {
+                const uint32_t c = atomicAdd(&ncand, 1u);
+                if (c < kRefineCap) cand[c] = i0 + k * NT;
+            }
+    }
+    __syncthreads();
+    uint32_t nc = ncand;
+    if (nc > (uint32_t)kRefineCap || mi == 0xffffffffu) {
+        nc = 1;
+        __syncthreads();
+        if (tid == 0) cand[0] = mi == 0xffffffffu ? 0u : mi;
+        __syncthreads();
+    }
+    // exact values; arg-max with the lowest index on ties.  Candidates whose bounding
+    // box fits one box evaluation take the staged path, others one chain each.
+    if (tid == 0) {
+        bb[0] = bb[1] = 1 << 30;
+        bb[2] = bb[3] = -1;
+    }
+    __syncthreads();
+    for (uint32_t c = tid; c < nc; c += NT) {
+        const int cx = (int)(cand[c] % cross.x), cy = (int)(cand[c] / cross.x);
+        atomicMin(&bb[0], cx);
+        atomicMin(&bb[1], cy);
+        atomicMax(&bb[2], cx);
+        atomicMax(&bb[3], cy);
+    }
+    __syncthreads();
+    const int cbw = bb[2] - bb[0] + 1, cbh = bb[3] - bb[1] + 1;
+    const bool boxed = cbh <= kBoxRows && cbw * cbh <= NT;
+    if (boxed) eval_box<T, NT>(ps, rs, s, cross, bb[0], bb[1], cbw, cbh, rrow, prow, boxv);
+    T best = lowest<T>();
+    uint32_t bi = 0xffffffffu;
+    for (uint32_t c = tid; c < nc; c += NT) {
+        const uint32_t i = cand[c];
+        const int cx = (int)(i % cross.x), cy = (int)(i / cross.x);
+        const T v = boxed ? boxv[(cy - bb[1]) * cbw + (cx - bb[0])]
+                          : exact_at<T>((size_t)sn * area + i, corr_len, p, r, s, cross, nb);
+        if (v > best || (v == best && i < bi)) {
+            best = v;
+            bi = i;
+        }
+    }
+    block_argmax<T, NT>(best, bi, sv, si);
+
+    // neighbourhood, exact: the in-window part as one box, positions the clamp puts
+    // one past the window (spilling into the next row or slice) one chain each
+    const int k = fit * fit, rad = (fit - 1) / 2;
+    const size2 mp = shift_pos(POLICY_BRUTE, bi, cross);
+    const int nx0 = max(0, (int)mp.x - rad), ny0 = max(0, (int)mp.y - rad);
+    const int nbw = min((int)cross.x - 1, (int)mp.x + rad) - nx0 + 1;
+    const int nbh = min((int)cross.y - 1, (int)mp.y + rad) - ny0 + 1;
+    eval_box<T, NT>(ps, rs, s, cross, nx0, ny0, nbw, nbh, rrow, prow, boxv);
+    for (int t = tid; t < k; t += NT) {
+        const int i = t % fit, j = t / fit;
+        const size2 c = clamp_pos((int)mp.x - rad + i, (int)mp.y - rad + j, cross);
+        const T v = (c.x < cross.x && c.y < cross.y)
+                        ? boxv[((int)c.y - ny0) * nbw + ((int)c.x - nx0)]
+                        : exact_at<T>(pos_index(POLICY_BRUTE, sn, c, cross), corr_len, p, r, s, cross, nb);
+        nbv[t] = v;
+        if (neigh) neigh[(size_t)sn * k + t] = v;
+    }
+    __syncthreads();
+    if (tid < 6) cf[tid] = fit_coef<T>(lsq + tid * k, nbv, k);
+    __syncthreads();
+    if (tid == 0) {
+        if (maxes) {
+            maxes[sn].data = best;
+            maxes[sn].index = bi;
+        }
+        double px, py, ox, oy;
+        fit_peak(cf, px, py);
+        finalize_offset(bi, cross, POLICY_BRUTE, fit, px, py, ox, oy);
+        if (offsets) {
+            offsets[2 * (size_t)sn] = ox;
+            offsets[2 * (size_t)sn + 1] = oy;
+        }
+        if (coefs)
+            for (int c = 0; c < 6; ++c) coefs[6 * (size_t)sn + c] = cf[c];
+    }
+}
+
 // --------------------------------------------------------------- launchers
 constexpr int kPeakNT = 256;
 
@@ -294,6 +562,24 @@ void launch_subpixel(const T* neigh, int s, uint32_t n, const data_index<T>* max
     subpixel_kernel<T><<<div_up(n, 64u), 64, 0, st>>>(neigh, s, n, lsq, maxes, cross, policy, offsets, coefs);
     EMIDA_LAUNCHED("subpixel_kernel");
 }
+
+template <class T>
+void launch_refine(const float* screen, size_t corr_len, const T* p, const T* r, size2 s, size2 cross, uint32_t nb,
+                   double tol_rel, double tol_abs, int fit, data_index<T>* maxes, T* neigh, double* offsets,
+                   double* coefs, uint32_t total, hipStream_t st) {
+    if (total == 0) return;
+    const double* lsq = lstsq_device_table() + lstsq_offset(fit);
+    const size_t lds = (size_t)(2 * kBoxY + kBoxRows - 1) * (s.x + 1) * sizeof(T);
+    refine_kernel<T, kRefineNT><<<total, kRefineNT, lds, st>>>(screen, corr_len, p, r, s, cross, nb, tol_rel, tol_abs,
+                                                          fit, lsq, maxes, neigh, offsets, coefs, total);
+    EMIDA_LAUNCHED("refine_kernel");
+}
+template void launch_refine<float>(const float*, size_t, const float*, const float*, size2, size2, uint32_t, double,
+                                   double, int, data_index<float>*, float*, double*, double*, uint32_t,
+                                   hipStream_t);
+template void launch_refine<double>(const float*, size_t, const double*, const double*, size2, size2, uint32_t,
+                                    double, double, int, data_index<double>*, double*, double*, double*, uint32_t,
+                                    hipStream_t);
 
 #define INST_PEAK(T)                                                                                          \
     template void launch_peak<T>(const T*, size_t, uint32_t, size2, int, int, data_index<T>*, T*, double*,     \

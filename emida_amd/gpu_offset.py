@@ -97,13 +97,14 @@ class GpuOffset:
     T: np.float32 or np.float64 (correlation precision); IN: np.uint16, np.float32
     or np.float64 (pixel type); begins: list of (x, y) ROI corners (copied);
     slice_size, cross_size, src_size: (x, y); policy: CROSS_POLICY_BRUTE,
-    CROSS_POLICY_FFT or CROSS_POLICY_BRUTE_MFMA (matrix-core brute correlation,
-    T must be float32); s: fitsize 3, 5, 7 or 9; mfma_mode: MFMA_BF16,
-    MFMA_BF16_SPLIT, MFMA_F16 or MFMA_F16_SPLIT (used by CROSS_POLICY_BRUTE_MFMA).
+    CROSS_POLICY_FFT or CROSS_POLICY_BRUTE_MFMA (matrix-core brute correlation);
+    s: fitsize 3, 5, 7 or 9; mfma_mode: MFMA_BF16, MFMA_BF16_SPLIT, MFMA_F16 or
+    MFMA_F16_SPLIT, and mfma_refine: exact peak equal to CROSS_POLICY_BRUTE's results
+    (False: float screen values, T must be float32); both for CROSS_POLICY_BRUTE_MFMA.
     """
 
     def __init__(self, T, IN, src_size, begins, slice_size, cross_size, batch_size, policy=CROSS_POLICY_BRUTE,
-                 s=3, in_buffers_size=1, mfma_mode=MFMA_F16_SPLIT):
+                 s=3, in_buffers_size=1, mfma_mode=MFMA_F16_SPLIT, mfma_refine=True):
         self.T = np.dtype(T)
         self.IN = np.dtype(IN)
         self.src_size = tuple(int(v) for v in src_size)
@@ -121,6 +122,7 @@ class GpuOffset:
         self._h = h
         if self.policy == CROSS_POLICY_BRUTE_MFMA:
             C.check(C.lib().emida_offset_set_mfma_mode(h, int(mfma_mode)))
+            C.check(C.lib().emida_offset_set_mfma_refine(h, 1 if mfma_refine else 0))
         self.mfma_mode = int(mfma_mode)
         info = C.OffsetInfo()
         C.check(C.lib().emida_offset_get_info(h, ctypes.byref(info)))

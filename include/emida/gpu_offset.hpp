@@ -58,7 +58,7 @@ using size2_t = vec2<esize_t>;
 enum cross_policy {
     CROSS_POLICY_BRUTE = EMIDA_CROSS_POLICY_BRUTE,
     CROSS_POLICY_FFT = EMIDA_CROSS_POLICY_FFT,
-    // Matrix-core brute correlation (no reference counterpart; T = float only).
+    // Matrix-core brute correlation (no reference counterpart).
     CROSS_POLICY_BRUTE_MFMA = EMIDA_CROSS_POLICY_BRUTE_MFMA
 };
 
@@ -120,7 +120,6 @@ public:
         emida_offset_info info;
         detail::check(emida_offset_get_info(h_, &info), "gpu_offset");
         total_ = info.total_slices;
-
         neigh_ = info.neigh_size;
     }
     gpu_offset(const gpu_offset&) = delete;
@@ -142,6 +141,10 @@ public:
     // Half type of CROSS_POLICY_BRUTE_MFMA; call before allocate_memory (no reference counterpart).
     void set_mfma_mode(emida_mfma_mode mode) {
         detail::check(emida_offset_set_mfma_mode(h_, mode), "set_mfma_mode");
+    }
+    // Exact peak equal to CROSS_POLICY_BRUTE's results (default); before allocate_memory.
+    void set_mfma_refine(bool enable) {
+        detail::check(emida_offset_set_mfma_refine(h_, enable ? 1 : 0), "set_mfma_refine");
     }
 
     // src/gpu_offset.hpp:127

@@ -41,8 +41,9 @@ typedef enum { EMIDA_F32 = 0, EMIDA_F64 = 1, EMIDA_I32 = 2, EMIDA_U16 = 3 } emid
 
 /* cross_policy, src/common.hpp:141-145.  BRUTE_MFMA (no reference counterpart) is
  * the brute-force correlation as an implicit GEMM on BF16/FP16 matrix cores with
- * FP32 accumulation (precision set by emida_offset_set_mfma_mode); it requires
- * T = F32 and a slice width that is a multiple of 8 up to 128. */
+ * FP32 accumulation (precision set by emida_offset_set_mfma_mode), refined to the
+ * BRUTE policy's results by emida_offset_set_mfma_refine; it requires a slice width
+ * that is a multiple of 8 up to 128. */
 typedef enum {
     EMIDA_CROSS_POLICY_BRUTE = 0,
     EMIDA_CROSS_POLICY_FFT = 1,
@@ -153,6 +154,14 @@ int emida_offset_process_device(emida_offset* h, const void* d_pics, uint32_t n_
 /* Operand precision of the BRUTE_MFMA policy (default EMIDA_MFMA_F16_SPLIT); call
  * before emida_offset_allocate_memory.  No reference counterpart. */
 int emida_offset_set_mfma_mode(emida_offset* h, emida_mfma_mode mode);
+
+/* Exact peak for the BRUTE_MFMA policy (default on): the matrix-core correlation only
+ * screens; every shift within its rounding bound of the screen maximum and the F x F
+ * neighbourhood are evaluated the way the BRUTE policy evaluates them, in T, so the
+ * results equal the BRUTE policy's (T = F32 or F64).  Off (T = F32 only): the float
+ * screen values are used directly.  Call before emida_offset_allocate_memory.  No
+ * reference counterpart. */
+int emida_offset_set_mfma_refine(emida_offset* h, int enable);
 
 /* Kernel timing with HIP events on the stream the kernels run on.  Returns the
  * summed time and launch count of the dominant kernel (the FFT pipeline kernel

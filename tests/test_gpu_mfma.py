@@ -120,33 +120,34 @@ def well_posed(coefs, fit):
     return (d < 0) & (det > 0) & (px >= 0) & (px <= fit - 1) & (py >= 0) & (py <= fit - 1)
 
 
-def _mfma_offsets(ps, begins, slice_size, cross, fit, batch, mode):
-    with emida_amd.GpuOffset(np.float32, np.uint16, ps.size, begins, slice_size, cross, batch,
-                             emida_amd.CROSS_POLICY_BRUTE_MFMA, fit, mfma_mode=mode) as g:
+def _run(ps, begins, slice_size, cross, fit, batch, T=np.float32, policy=emida_amd.CROSS_POLICY_BRUTE_MFMA,
+         mode=emida_amd.MFMA_F16_SPLIT, refine=True):
+    with emida_amd.GpuOffset(T, np.uint16, ps.size, begins, slice_size, cross, batch, policy, fit, mfma_mode=mode,
+                             mfma_refine=refine) as g:
         g.allocate_memory(ps.ref)
         maxes, neigh = g.new_result_buffers(pinned=False)
         idx = g.transfer_pic_to_device_async(ps.pics[:batch])
         g.wait_uploads()
         g.get_offset_core(maxes, neigh, idx)
-        offs, _ = g.finalize(maxes, neigh)
+        offs, coefs = g.finalize(maxes, neigh)
         p_offs, _ = g.get_offset(ps.pics[:batch])
         np.testing.assert_array_equal(offs, p_offs)  # both host forms, same results
-        return offs
+        return offs, coefs, maxes.copy(), neigh.copy()
 
 
-# max offset error vs FP64 on well-posed ROIs (measured on MI355X: bf16 2.4e-3,
-# bf16 split 3.5e-5, fp16 2.8e-4, fp16 split 2.9e-5 px); every mode meets 0.01 px
+# max offset error vs FP64 on well-posed ROIs of the unrefined float screen (measured
+# on MI355X: bf16 2.4e-3, bf16 split 3.5e-5, fp16 2.8e-4, fp16 split 2.9e-5 px)
 PIPE_TOL = {emida_amd.MFMA_BF16: 0.01, emida_amd.MFMA_BF16_SPLIT: 5e-4, emida_amd.MFMA_F16: 2e-3,
             emida_amd.MFMA_F16_SPLIT: 5e-4}
 
 
 @pytest.mark.parametrize("mode", list(MODES))
-def test_pipeline_vs_fp64_oracle(mode):
+def test_screen_only_pipeline_vs_fp64_oracle(mode):
     ps = synth.PatternSet("mfma", 256, 3, t_range=2.0)
     begins = synth.grid_begins(256, 4, 32)
     eo, ec = O.get_offset(ps.pics.ravel(), ps.ref, ps.size, begins, (64, 64), (127, 127), 3, 7, O.POLICY_BRUTE,
                           np.float64)
-    offs = _mfma_offsets(ps, begins, (64, 64), (127, 127), 7, 3, mode)
+    offs = _run(ps, begins, (64, 64), (127, 127), 7, 3, mode=mode, refine=False)[0]
     ok = well_posed(ec, 7)
     assert ok.mean() > 0.5
     err = np.abs(offs - eo)[ok].max()
@@ -154,15 +155,43 @@ def test_pipeline_vs_fp64_oracle(mode):
     assert err < PIPE_TOL[mode]
 
 
+# Refined: every output equals the brute policy's in the same precision, bit for bit,
+# whatever the screen's precision (even a single bf16 half).
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+@pytest.mark.parametrize("mode", list(MODES))
+def test_refined_equals_brute_policy(T, mode):
+    ps = synth.PatternSet("mfma", 256, 3, t_range=2.0)
+    begins = synth.grid_begins(256, 4, 32)
+    got = _run(ps, begins, (64, 64), (127, 127), 7, 3, T=T, mode=mode)
+    exp = _run(ps, begins, (64, 64), (127, 127), 7, 3, T=T, policy=emida_amd.CROSS_POLICY_BRUTE)
+    for g, e, what in zip(got, exp, ("offsets", "coefs", "maxes", "neighbourhoods")):
+        np.testing.assert_array_equal(g, e, err_msg=what)
+
+
+@pytest.mark.parametrize("cross", [(9, 9), (15, 7)])
+def test_refined_equals_brute_policy_edge_peaks(cross):
+    # displacements up to 3 px in a small window put maxima on the window edge: the
+    # neighbourhood clamp reads one past the edge, into the next row or slice
+    ps = synth.PatternSet("mfma-edges", 256, 2, t_range=3.0)
+    begins = synth.grid_begins(256, 4, 16)
+    got = _run(ps, begins, (32, 32), cross, 3, 2, T=np.float64, mode=emida_amd.MFMA_BF16)
+    exp = _run(ps, begins, (32, 32), cross, 3, 2, T=np.float64, policy=emida_amd.CROSS_POLICY_BRUTE)
+    for g, e in zip(got, exp):
+        np.testing.assert_array_equal(g, e)
+
+
 @pytest.mark.slow
 def test_precision_form_workload_subset_vs_fp64_oracle():
     # the precision-form workload's shape: 873 x 873 patterns, ROIs of 128 x 128 on a
-    # 10 x 10 grid, full 255 x 255 correlation, fitsize 7 (25 of the 100 ROIs)
+    # 10 x 10 grid, full 255 x 255 correlation, fitsize 7 (25 of the 100 ROIs): refined
+    # double results are the FP64 oracle's, screen-only split modes within 0.01 px
     ps = synth.PatternSet("mfma873", 873, 1, t_range=3.0)
     begins = synth.grid_begins(873, 10, 64)[::4]
     eo, ec = O.get_offset(ps.pics.ravel(), ps.ref, ps.size, begins, (128, 128), (255, 255), 1, 7, O.POLICY_BRUTE,
                           np.float64)
+    offs = _run(ps, begins, (128, 128), (255, 255), 7, 1, T=np.float64)[0]
+    np.testing.assert_array_equal(offs, eo)
     ok = well_posed(ec, 7)
     for mode in (emida_amd.MFMA_BF16_SPLIT, emida_amd.MFMA_F16_SPLIT):
-        offs = _mfma_offsets(ps, begins, (128, 128), (255, 255), 7, 1, mode)
+        offs = _run(ps, begins, (128, 128), (255, 255), 7, 1, mode=mode, refine=False)[0]
         assert np.abs(offs - eo)[ok].max() < 0.01

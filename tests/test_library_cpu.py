@@ -59,7 +59,7 @@ def test_create_valid_configurations_without_gpu():
     (dict(policy=emida_amd.CROSS_POLICY_FFT, slice_size=(128, 128)), C.ERR_UNSUPPORTED),  # double: <= 64
     (dict(policy=emida_amd.CROSS_POLICY_FFT, T=np.float32, slice_size=(256, 256), src_size=(300, 300)),
      C.ERR_UNSUPPORTED),
-    (dict(policy=emida_amd.CROSS_POLICY_BRUTE_MFMA), C.ERR_UNSUPPORTED),           # T must be float
+    (dict(policy=emida_amd.CROSS_POLICY_BRUTE_MFMA, mfma_refine=False), C.ERR_ARG),  # float screen, T double
     (dict(policy=emida_amd.CROSS_POLICY_BRUTE_MFMA, T=np.float32, slice_size=(36, 32)), C.ERR_UNSUPPORTED),
     (dict(policy=emida_amd.CROSS_POLICY_BRUTE_MFMA, T=np.float32, mfma_mode=7), C.ERR_ARG),
     (dict(policy=7), C.ERR_ARG),
@@ -80,6 +80,8 @@ def test_create_matrix_core_policy_without_gpu():
     for mode in (emida_amd.MFMA_BF16, emida_amd.MFMA_BF16_SPLIT, emida_amd.MFMA_F16, emida_amd.MFMA_F16_SPLIT):
         g = _create(T=np.float32, policy=emida_amd.CROSS_POLICY_BRUTE_MFMA, mfma_mode=mode, cross_size=(21, 41))
         assert g.cross_size == (21, 41) and g.neigh_size == 9 and g.mfma_mode == mode
+        g = _create(T=np.float64, policy=emida_amd.CROSS_POLICY_BRUTE_MFMA, mfma_mode=mode)  # refined: double
+        assert g.data_index.itemsize == 16
 
 
 def test_synthetic_grid_matches_benchmark_definition():

@@ -84,51 +84,62 @@ __global__ __launch_bounds__(256) void prepare_split_kernel(const IN* __restrict
 
 constexpr int kMfmaWaves = 4;
 constexpr int kMfmaTilesPerWave = 2;
-constexpr int kMfmaSX = kMfmaWaves * kMfmaTilesPerWave * 32;  // x-shifts per workgroup (256)
-constexpr int kMfmaM = 32;                                    // patterns per workgroup
+constexpr int kMfmaM = 32;  // patterns per workgroup
 constexpr int kMfmaNT = kMfmaWaves * 64;
+constexpr int kMfmaTS = 4;  // y-shifts per wave
 
-template <class HT, bool SPLIT, int TS>
+// Geometry for pattern rows of at most KMAX columns: WX waves side by side along the
+// x-shifts (two 32-wide tiles each, 64 WX x-shifts per workgroup) and 4 / WX waves
+// along the y-shifts (4 each).  KMAX = 128 pairs with WX = 4 (windows up to 256
+// wide), 64 with 2 (up to 128), 32 with 1 (up to 64): every wave has work and the
+// LDS ring stays at most 64 KiB per plane pair.
+template <class HT, bool SPLIT, int KMAX, int WX>
 struct mfma_plan {
-    static constexpr int KMAX = 128;                 // slice width limit (K); pattern row length in LDS
+    static constexpr int WY = kMfmaWaves / WX;
+    static constexpr int ROWS = kMfmaTS * WY;        // y-shifts per workgroup = ring slots
+    static constexpr int SXW = WX * kMfmaTilesPerWave * 32;  // x-shifts per workgroup
     static constexpr int CHMAX = KMAX / 8;           // 16-byte chunks per pattern row
-    static constexpr int RP = KMAX + kMfmaSX + 8;    // length of one shifted reference copy
+    static constexpr int SWZ = 128 / KMAX;           // rows per 256-byte LDS sweep
+    static constexpr int RP = KMAX + SXW + 8;        // length of one shifted reference copy
     static constexpr int NH = SPLIT ? 2 : 1;         // hi (+ lo) planes
-    static constexpr size_t ring = (size_t)TS * kMfmaM * KMAX;  // elements of one plane of the row ring
+    static constexpr size_t ring = (size_t)ROWS * kMfmaM * KMAX;  // elements of one plane of the row ring
     static constexpr size_t lds_a = NH * ring * sizeof(HT);
     static constexpr size_t lds_r = (size_t)NH * 8 * RP * sizeof(HT);
     static constexpr size_t lds_bytes = lds_a + lds_r;
     static constexpr int PF = (kMfmaM * CHMAX + kMfmaNT - 1) / kMfmaNT;  // prefetched row chunks per thread
+    // 16-byte chunk c of pattern row m is stored at chunk swz(c, m): the 16 rows one
+    // fragment read touches fall on 16 distinct 16-byte bank groups
+    __device__ static int swz(int c, int m) { return c ^ ((m / SWZ) & (CHMAX - 1)); }
 };
 
 // hi/lo: prepared pattern slices [slice][y][x] (slice = pattern * nb + roi);
 // rhi/rlo: prepared reference slices [roi][y][x]; corr: [slice][cross.y][cross.x].
-// n_wg = nb * ceil(npat / 32) * ceil(cross.y / TS) workgroups (1-D grid).
-template <class HT, bool SPLIT, int TS>
+// n_wg = nb * ceil(npat / 32) * ceil(cross.y / ROWS) workgroups (1-D grid).
+template <class HT, bool SPLIT, int KMAX, int WX>
 __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restrict__ phi,
                                                                 const HT* __restrict__ plo,
                                                                 const HT* __restrict__ rhi,
                                                                 const HT* __restrict__ rlo,
                                                                 float* __restrict__ corr, size2 s, size2 cross,
                                                                 uint32_t nb, uint32_t npat, uint32_t n_wg) {
-    using P = mfma_plan<HT, SPLIT, TS>;
+    using P = mfma_plan<HT, SPLIT, KMAX, WX>;
     using V = typename mfma_vec<HT>::type;
-    constexpr int NT = kMfmaNT;
+    constexpr int NT = kMfmaNT, TS = kMfmaTS, ROWS = P::ROWS;
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    // A: [NH][TS][32][KMAX] pattern-row ring, 16-byte chunk c of row m stored at chunk
-    //    c ^ (m & 15) (conflict-free fragment reads without padding);
+    // A: [NH][ROWS][32][KMAX] pattern-row ring, 16-byte chunk c of row m stored at
+    //    chunk P::swz(c, m) (conflict-free fragment reads without padding);
     // Rs: [NH][8][RP] shifted reference copies.
     HT* A = reinterpret_cast<HT*>(lds_raw);
     HT* Rs = reinterpret_cast<HT*>(lds_raw + P::lds_a);
 
     const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-    const uint32_t n_syt = div_up(cross.y, (uint32_t)TS);
+    const uint32_t n_syt = div_up(cross.y, (uint32_t)ROWS);
     const uint32_t L = xcd_remap(blockIdx.x, n_wg);  // y-shift tiles of one (ROI, pattern tile) back to back
     const uint32_t syt = L % n_syt, slab = L / n_syt;
     const uint32_t roi = slab % nb, mt = slab / nb;
     const int rx = (int)(cross.x - 1) / 2, ry = (int)(cross.y - 1) / 2;
-    const int sy0 = (int)syt * TS - ry;
-    const int off = 256 - rx;  // shifted-copy offset, see below
+    const int sy0 = (int)syt * ROWS - ry;  // first y-shift of the workgroup
+    const int off = P::SXW - rx;            // shifted-copy offset, see below
     const int S = (int)s.x, SY = (int)s.y, CH = S / 8;
     const int K16 = (S + 15) / 16 * 16;
     const uint32_t p0 = mt * kMfmaM;
@@ -136,9 +147,11 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
     const HT* refh = rhi + (size_t)roi * slice_elems;
     const HT* refl = SPLIT ? rlo + (size_t)roi * slice_elems : nullptr;
 
-    // this wave's two adjacent 32-wide x-shift tiles; the pair index rotates with the
-    // y-shift tile so that light (edge) and heavy (centre) pairs alternate per SIMD
-    const int pair = (wave + (int)syt) % kMfmaWaves;
+    // this wave's two adjacent 32-wide x-shift tiles (the pair index rotates with the
+    // y-shift tile so that light edge and heavy centre pairs alternate per SIMD) and
+    // its 4 y-shifts sy0 + 4 wy + t
+    const int pair = (wave % WX + (int)syt) % WX, wy = wave / WX;
+    const int syw = sy0 + TS * wy;
     int sx0[kMfmaTilesPerWave];
     bool active[kMfmaTilesPerWave];
 #pragma unroll
@@ -180,13 +193,13 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
     };
     auto store_prow = [&](int Y) {
         if (Y < 0 || Y >= SY) return;  // rows outside the slice are never read
-        const int slot = Y % TS;
+        const int slot = Y % ROWS;
 #pragma unroll
         for (int i = 0; i < P::PF; ++i) {
             const int e = tid + NT * i;
             if (e < kMfmaM * CH) {
                 const int m = e / CH, c = e % CH;
-                const size_t o = ((size_t)slot * kMfmaM + m) * P::KMAX + 8 * (c ^ (m & 15));
+                const size_t o = ((size_t)slot * kMfmaM + m) * KMAX + 8 * P::swz(c, m);
                 *reinterpret_cast<V*>(A + o) = pfh[i];
                 if constexpr (SPLIT) *reinterpret_cast<V*>(A + P::ring + o) = pfl[i];
             }
@@ -194,7 +207,7 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
     };
     // reference row y as 8 shifted copies: Rs[rho][i] = R[y][i + rho - off] (0 outside
     // the row), so the window R[y][q .. q+7] is Rs[rho][jp - rho ..] with jp = q + off,
-    // rho = jp & 7.  q spans [rx - 255, K16 - 8 + rx], hence jp in [1, K16 + 248].
+    // rho = jp & 7.  q spans [rx - SXW + 1, K16 - 8 + rx], hence jp in [1, K16 + SXW - 8].
     HT rfh = HT(0.0f), rfl = HT(0.0f);
     auto load_rrow = [&](int y) {
         if (tid < S) {
@@ -213,23 +226,23 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
         }
     };
 
-    // y range: some y-shift of the tile overlaps reference row y
-    const int y_lo = max(0, -(sy0 + TS - 1)), y_hi = min(SY, SY - sy0);
-    for (int Y = y_lo + sy0; Y < y_lo + sy0 + TS - 1; ++Y) {
+    // y range: some y-shift of the workgroup overlaps reference row y
+    const int y_lo = max(0, -(sy0 + ROWS - 1)), y_hi = min(SY, SY - sy0);
+    for (int Y = y_lo + sy0; Y < y_lo + sy0 + ROWS - 1; ++Y) {
         load_prow(Y);
         store_prow(Y);
     }
     if (y_lo < y_hi) {
-        load_prow(y_lo + sy0 + TS - 1);
+        load_prow(y_lo + sy0 + ROWS - 1);
         load_rrow(y_lo);
     }
     const int m_a = lane % 32, half = lane / 32;
     for (int y = y_lo; y < y_hi; ++y) {
-        store_prow(y + sy0 + TS - 1);
+        store_prow(y + sy0 + ROWS - 1);
         store_rrow();
         __syncthreads();
         if (y + 1 < y_hi) {
-            load_prow(y + sy0 + TS);
+            load_prow(y + sy0 + ROWS);
             load_rrow(y + 1);
         }
         for (int k0 = 0; k0 < K16; k0 += 16) {
@@ -252,12 +265,12 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
                 bh[u] = *reinterpret_cast<const V*>(Rs + rho * P::RP + (jp - rho));
                 if constexpr (SPLIT) bl[u] = *reinterpret_cast<const V*>(Rs + (8 + rho) * P::RP + (jp - rho));
             }
-            const int cs = 8 * ((k0 / 8 + half) ^ (m_a & 15));
+            const int cs = 8 * P::swz(k0 / 8 + half, m_a);
 #pragma unroll
             for (int t = 0; t < TS; ++t) {
-                const int Y = y + sy0 + t;
+                const int Y = y + syw + t;
                 if (Y < 0 || Y >= SY) continue;
-                const HT* arow = A + ((size_t)(Y % TS) * kMfmaM + m_a) * P::KMAX + cs;
+                const HT* arow = A + ((size_t)(Y % ROWS) * kMfmaM + m_a) * KMAX + cs;
                 const V ah = *reinterpret_cast<const V*>(arow);
                 V al{};
                 if constexpr (SPLIT) al = *reinterpret_cast<const V*>(arow + P::ring);
@@ -277,7 +290,7 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
     // write the accumulators: D[8 (j/4) + 4 (l/32) + j%4][l%32]
 #pragma unroll
     for (int t = 0; t < TS; ++t) {
-        const int sy = sy0 + t;
+        const int sy = syw + t;
         if (sy + ry >= (int)cross.y) continue;
 #pragma unroll
         for (int u = 0; u < kMfmaTilesPerWave; ++u) {
@@ -296,21 +309,33 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
 }
 
 // --------------------------------------------------------------- launchers
-constexpr int kMfmaTS = 4;
-
-template <class HT, bool SPLIT>
-void launch_mfma_corr_t(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
+template <class HT, bool SPLIT, int KMAX, int WX>
+void launch_mfma_corr_g(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
                         uint32_t nb, uint32_t npat, hipStream_t st) {
-    using P = mfma_plan<HT, SPLIT, kMfmaTS>;
-    auto kern = mfma_corr_kernel<HT, SPLIT, kMfmaTS>;
+    using P = mfma_plan<HT, SPLIT, KMAX, WX>;
+    static_assert(P::lds_bytes <= 80 * 1024, "two workgroups per CU");
+    auto kern = mfma_corr_kernel<HT, SPLIT, KMAX, WX>;
     static const hipError_t attr =
         hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P::lds_bytes);
     EMIDA_HIP(attr);
-    const uint64_t n_wg = (uint64_t)nb * div_up(npat, (uint32_t)kMfmaM) * div_up(cross.y, (uint32_t)kMfmaTS);
+    if (s.x > (uint32_t)KMAX || cross.x > (uint32_t)P::SXW) throw arg_error("matrix-core geometry mismatch");
+    const uint64_t n_wg = (uint64_t)nb * div_up(npat, (uint32_t)kMfmaM) * div_up(cross.y, (uint32_t)P::ROWS);
     if (n_wg > (1u << 24)) throw unsupported_error("matrix-core brute policy: too many workgroups in one launch");
     kern<<<(uint32_t)n_wg, kMfmaNT, P::lds_bytes, st>>>(phi, plo, rhi, rlo, corr, s, cross, nb, npat,
                                                         (uint32_t)n_wg);
     EMIDA_LAUNCHED("mfma_corr_kernel");
+}
+
+// Row width class of a slice: 32, 64 or 128 columns (windows up to 64, 128, 256 wide).
+template <class HT, bool SPLIT>
+void launch_mfma_corr_t(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
+                        uint32_t nb, uint32_t npat, hipStream_t st) {
+    if (s.x <= 32)
+        launch_mfma_corr_g<HT, SPLIT, 32, 1>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st);
+    else if (s.x <= 64)
+        launch_mfma_corr_g<HT, SPLIT, 64, 2>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st);
+    else
+        launch_mfma_corr_g<HT, SPLIT, 128, 4>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st);
 }
 
 void check_mfma_shape(size2 s, size2 cross) {

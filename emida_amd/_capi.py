@@ -59,6 +59,17 @@ class ServerStats(ctypes.Structure):
                 ("device_bytes_peak_inflight", ctypes.c_uint64), ("slots", ctypes.c_uint32)]
 
 
+class FilesStats(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_uint32), ("failed_batches", ctypes.c_uint32), ("wall_ms", ctypes.c_double),
+                ("load_ms", ctypes.c_double), ("upload_wait_ms", ctypes.c_double)]
+
+
+# void (*)(void* user, uint32 first, uint32 count, int status, const char* error,
+#          const double* offsets, const double* coefs)
+FILES_CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                  ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double))
+
+
 class OffsetInfo(ctypes.Structure):
     _fields_ = [("total_slices", ctypes.c_uint32), ("neigh_size", ctypes.c_uint32),
                 ("data_index_bytes", ctypes.c_uint32), ("cross_size", Size2), ("corr_buffer", Size2),
@@ -126,6 +137,14 @@ def lib():
             "emida_server_wait": ([vp, u64, ctypes.POINTER(RequestInfo)], i32),
             "emida_server_get_stats": ([vp, ctypes.POINTER(ServerStats)], i32),
             "emida_generate_hanning": ([i32, sz, vp], i32),
+            "emida_tiff_size": ([ctypes.c_char_p, ctypes.POINTER(S)], i32),
+            "emida_tiff_load": ([ctypes.c_char_p, vp, S], i32),
+            "emida_process_files": ([vp, ctypes.POINTER(ctypes.c_char_p), u32, u32, i32, FILES_CALLBACK, vp,
+                                     ctypes.POINTER(FilesStats)], i32),
+            "emida_read_slicepos": ([ctypes.c_char_p, ctypes.POINTER(u64), ctypes.POINTER(i32), ctypes.POINTER(S), u32,
+                                     ctypes.POINTER(u32)], i32),
+            "emida_read_deformed_list": ([ctypes.c_char_p, dp, u32, ctypes.POINTER(u32), ctypes.c_char_p, sz,
+                                          ctypes.POINTER(sz)], i32),
         }
         for name, (args, res) in sigs.items():
             f = getattr(L, name)

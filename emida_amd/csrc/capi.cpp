@@ -53,9 +53,11 @@ int emida_offset_create(emida_offset** out, emida_type T, emida_type IN, emida_s
             // the FFT policy forces cross_size = 2s - 1 (src/gpu_offset.hpp:114)
             if (slice_size.x != slice_size.y)
                 throw unsupported_error("FFT policy: square slices only");
+            // any side up to half the largest transform; non-powers of two are
+            // zero-padded to the next power of two (same correlation values)
             const uint32_t s = slice_size.x;
-            if ((s & (s - 1)) != 0 || s > (uint32_t)fft_max_side(T == EMIDA_F64) / 2)
-                throw unsupported_error("FFT policy: slice side must be a power of two <= " +
+            if (s < 2 || s > (uint32_t)fft_max_side(T == EMIDA_F64) / 2)
+                throw unsupported_error("FFT policy: slice side must be between 2 and " +
                                         std::to_string(fft_max_side(T == EMIDA_F64) / 2));
             h->side = s;
             h->cross = {2 * s - 1, 2 * s - 1};
@@ -75,7 +77,7 @@ int emida_offset_get_info(const emida_offset* h, emida_offset_info* info) {
         info->data_index_bytes = (uint32_t)h->ti_bytes();
         info->cross_size = {h->cross.x, h->cross.y};
         info->corr_buffer = {h->buf.x, h->buf.y};
-        info->fft_size = h->side * 2;
+        info->fft_size = h->side ? fft_padded(h->side) : 0;
         info->device_bytes = h->device_bytes;
     });
 }
@@ -90,15 +92,7 @@ int emida_offset_allocate_memory(emida_offset* h, const void* temp) {
 int emida_offset_transfer_pic_to_device_async(emida_offset* h, const void* pic, uint32_t* buffer_index) {
     return guarded([&] {
         if (!h || !pic) throw arg_error("null argument");
-        h->require_allocated();
-        std::unique_lock<std::mutex> lock(h->upload_mu);
-        const uint32_t i = h->rr.fetch_add(1) % h->n_in;
-        h->upload_cv.wait(lock, [&] { return !h->pending[i]; });
-        h->pending[i] = 1;
-        EMIDA_HIP(hipStreamWaitEvent(h->in_stream, h->ev_done[i], 0));
-        EMIDA_HIP(hipMemcpyAsync(h->d_in[i], pic, (size_t)h->src.area() * h->batch * h->in_bytes(),
-                                 hipMemcpyHostToDevice, h->in_stream));
-        EMIDA_HIP(hipEventRecord(h->ev_up[i], h->in_stream));
+        const uint32_t i = h->upload(pic);
         if (buffer_index) *buffer_index = i;
     });
 }
@@ -197,7 +191,7 @@ int emida_offset_get_timing(emida_offset* h, const char** kernel_name, double* t
         h->tev.clear();
         if (kernel_name)
             *kernel_name = h->policy == EMIDA_CROSS_POLICY_FFT
-                               ? (h->side > 64 ? "fft256_cols_kernel+fft256_rows_kernel" : "fft_kernel")
+                               ? (fft_padded(h->side) > 128 ? "fft256_cols_kernel+fft256_rows_kernel" : "fft_kernel")
                                : (h->policy == EMIDA_CROSS_POLICY_BRUTE_MFMA
                                       ? "mfma_corr_kernel"
                                       : cross_corr_kernel_name(h->T == EMIDA_F64, h->slice, h->cross, h->nb, h->batch));

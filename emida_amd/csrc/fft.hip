@@ -9,7 +9,15 @@
 // kernels with every intermediate in global memory; here nothing but the inputs, the
 // reference spectra and the per-ROI results touch HBM.
 //
-// FFT scheme.  A length-M transform (M = 2s) is computed by a group of L lanes of
+// Transform size.  The reference transforms the zero-padded 2s x 2s slice.  Here M is
+// the power of two >= 2s; the padding beyond 2s changes no correlation value (shifts
+// |d| < s are the linear correlation for any M >= 2s - 1), and the arg-max, the
+// gather and the optional result buffer read the M x M circular result through the
+// reference's 2s x 2s layout: its index k <= s is circular position k, k > s is
+// position k + M - 2s (shift k - 2s).  For power-of-two slices M = 2s and both
+// layouts coincide.
+//
+// FFT scheme.  A length-M transform is computed by a group of L lanes of
 // one wave, each lane holding R = M/L values: an R-point DFT in registers over
 // n = L*n1 + n2 (lane n2), a twiddle W_M^(n2*k1), one transpose through the group's
 // LDS scratch, and L-point DFTs in registers.  Lane j then holds X[k] for
@@ -17,20 +25,22 @@
 // inverse starts from that layout and ends in natural order, so the product with the
 // reference spectrum (stored in the same layout) needs no reordering.
 //
-// Stages for one ROI (S = M/2 = slice side, NC = S + 1 half-spectrum columns):
-//  0. load the S x S ROI, integer sum (uint16) or given sum, prepare
+// Stages for one ROI (s = slice side <= S = M/2, NC = S + 1 half-spectrum columns):
+//  0. load the s x s ROI, integer sum (uint16) or given sum, prepare
 //     ((px - mean) * hx * hy, the reference's order) into LDS;
-//  A. rows 2p and 2p+1 packed as one complex row: S/2 forward transforms;
+//  A. rows 2p and 2p+1 packed as one complex row: ceil(s/2) forward transforms;
 //  B. column kx of the half spectrum, split from the packed rows
 //     (X[k] and conj(X[M-k])); columns 0 and S are real and packed into one
 //     complex column; forward transform, product with conj(reference), inverse.
 //     Both split steps omit their factor 1/2, so column spectra come out doubled;
 //     the stored reference spectrum is scaled by 1/4 (half its true value), and the
-//     product is exact in scale (powers of two only);
+//     product is exact in scale (powers of two only); for a padded transform the
+//     stored spectrum also carries (2s/M)^2, which gives the unnormalised result the
+//     reference's (2s)^2 scale (one rounding of the stored spectrum, none when M = 2s);
 //  C. rows 2q and 2q+1 of the column result packed as Ga + i Gb (Hermitian
 //     extension, DC and Nyquist imaginary parts dropped as a C2R does), inverse
 //     transform yields both real correlation rows;
-//  D. arg-max over the (2s)^2 circular buffer, F x F gather through the FFT
+//  D. arg-max over the (2s)^2 circular buffer (reference layout), F x F gather through the FFT
 //     position policy, double-precision least-squares fit, finalize.
 #include "fft_common.hpp"
 
@@ -39,7 +49,7 @@ namespace emida_amd {
 // Geometry of one transform size.
 template <class T, int M>
 struct fft_plan {
-    static constexpr int S = M / 2;                  // slice side
+    static constexpr int S = M / 2;                  // largest slice side
     static constexpr int NC = S + 1;                 // half-spectrum columns
     static constexpr int L = M >= 64 ? 8 : (M >= 16 ? 4 : 2);  // lanes per transform
     static constexpr int R = M / L;                  // values per lane
@@ -81,6 +91,11 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
     const uint32_t bn = sn % a.nb;
     const int tid = threadIdx.x;
     const int lane = tid % L, grp = tid / L;
+    const int s = (int)a.side;   // slice side, 2 <= s <= S
+    const int pad = M - 2 * s;   // reference index k > s sits at circular position k + pad
+    // stored reference spectra: x 1/4 (the omitted split factors) x (2s/M)^2, so an
+    // M-point C2R carries the reference's (2s)^2 scale; s^2 / M^2 is exact in T
+    const T sc = (T)(0.25 * ((double)(2 * s) / M) * ((double)(2 * s) / M));
 
     // W_M^t for t < M, from double precision
     for (int t = tid; t < M; t += NT) {
@@ -98,7 +113,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
 #pragma unroll
         for (int it = 0; it < E; ++it) {
             const int e = tid + it * NT;
-            px[it] = e < S * S ? pic[(size_t)(e / S) * a.src.x + (e % S)] : IN(0);
+            px[it] = e < s * s ? pic[(size_t)(e / s) * a.src.x + (e % s)] : IN(0);
         }
         T mean;
         if constexpr (std::is_same<IN, uint16_t>::value) {
@@ -112,30 +127,31 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
             su = 0;
 #pragma unroll
             for (int k = 0; k < NT / kWave; ++k) su += red_u[k];
-            mean = mean_of<T>((T)su, S * S);
+            mean = mean_of<T>((T)su, s * s);
         } else {
-            mean = mean_of<T>((T)a.sums[sn], S * S);
+            mean = mean_of<T>((T)a.sums[sn], s * s);
         }
 #pragma unroll
         for (int it = 0; it < E; ++it) {
             const int e = tid + it * NT;
-            if (e < S * S) pre[e] = prep_px<T, IN>(px[it], mean, a.hx[e % S], a.hy[e / S]);
+            if (e < s * s) pre[e] = prep_px<T, IN>(px[it], mean, a.hx[e % s], a.hy[e / s]);
         }
     } else {
-        const T* src = a.prepared + (size_t)sn * S * S;
-        for (int e = tid; e < S * S; e += NT) pre[e] = src[e];
+        const T* src = a.prepared + (size_t)sn * s * s;
+        for (int e = tid; e < s * s; e += NT) pre[e] = src[e];
     }
     __syncthreads();
 
-    // ---- A. packed row pairs, forward
-    if (grp < S / 2) {
+    // ---- A. packed row pairs, forward (an odd s leaves the last pair's second row zero)
+    if (grp < (s + 1) / 2) {
         const int p = grp;
+        const bool second = 2 * p + 1 < s;
         C x[R];
 #pragma unroll
         for (int n1 = 0; n1 < R; ++n1) {
-            if (n1 < R / 2) {
-                const int n = L * n1 + lane;
-                x[n1] = C{pre[(2 * p) * S + n], pre[(2 * p + 1) * S + n]};
+            const int n = L * n1 + lane;
+            if (n1 < R / 2 && n < s) {
+                x[n1] = C{pre[(2 * p) * s + n], second ? pre[(2 * p + 1) * s + n] : T(0)};
             } else {
                 x[n1] = C{T(0), T(0)};
             }
@@ -155,8 +171,8 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
     if (c < S) {
 #pragma unroll
         for (int n1 = 0; n1 < R; ++n1) {
-            if (n1 < R / 2) {
-                const int y = L * n1 + lane;
+            const int y = L * n1 + lane;
+            if (n1 < R / 2 && y < s) {
                 const C* row = B + (y >> 1) * CS;
                 if (c == 0) {
                     const C z0 = row[0], zs = row[S];
@@ -206,8 +222,8 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
                 C* os = a.spectra_out + ((size_t)bn * P::NC + S) * M + lane * R;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    o0[r] = C{p0[r].re * T(0.25), p0[r].im * T(0.25)};
-                    os[r] = C{ps[r].re * T(0.25), ps[r].im * T(0.25)};
+                    o0[r] = C{p0[r].re * sc, p0[r].im * sc};
+                    os[r] = C{ps[r].re * sc, ps[r].im * sc};
                 }
             }
         } else {
@@ -218,7 +234,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
             } else {
                 C* oc = a.spectra_out + ((size_t)bn * P::NC + c) * M + lane * R;
 #pragma unroll
-                for (int r = 0; r < R; ++r) oc[r] = C{x[r].re * T(0.25), x[r].im * T(0.25)};
+                for (int r = 0; r < R; ++r) oc[r] = C{x[r].re * sc, x[r].im * sc};
             }
         }
         if constexpr (PIPE) {
@@ -267,43 +283,57 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
         __syncthreads();  // column results consumed; slots become row scratch
         // arg-max in two steps: the largest value (v_max, NaN ignored), then the
         // lowest index holding it, searched only by lanes that hold it
+        // over the positions of the reference layout only (all of them when M = 2s),
+        // with reference indices
+        const uint32_t s2 = 2 * s;
+        const int ya = 2 * q, yb = 2 * q + 1;
+        const bool in_a = ya <= s || ya > s + pad, in_b = yb <= s || yb > s + pad;
+        const uint32_t ra = (uint32_t)(ya <= s ? ya : ya - pad) * s2, rb = (uint32_t)(yb <= s ? yb : yb - pad) * s2;
         T vmax = -(T)__builtin_huge_val();
-        if (q < S) {
+        if (q < S && (in_a || in_b)) {
             C* slot = B + q * CS;
             fft_inv<T, R, L, M>(x, lane, slot, tw);
-            T* cout = a.corr_out ? a.corr_out + (size_t)sn * M * M : nullptr;
+            T* cout = a.corr_out ? a.corr_out + (size_t)sn * s2 * s2 : nullptr;
 #pragma unroll
             for (int n1 = 0; n1 < R; ++n1) {
                 const int xx = L * n1 + lane;
                 slot[xx] = x[n1];
-                vmax = fmax(vmax, fmax(x[n1].re, x[n1].im));
-                if (cout) {
-                    const uint32_t i0 = (uint32_t)(2 * q) * M + xx;
-                    cout[i0] = x[n1].re;
-                    cout[i0 + M] = x[n1].im;
+                if (xx <= s || xx > s + pad) {
+                    const uint32_t rx = (uint32_t)(xx <= s ? xx : xx - pad);
+                    if (in_a) vmax = fmax(vmax, x[n1].re);
+                    if (in_b) vmax = fmax(vmax, x[n1].im);
+                    if (cout) {
+                        if (in_a) cout[ra + rx] = x[n1].re;
+                        if (in_b) cout[rb + rx] = x[n1].im;
+                    }
                 }
             }
         }
         const T best = block_max<T, NT>(vmax, sv);
         uint32_t bi = 0xffffffffu;
-        if (q < S && vmax == best) {
+        if (q < S && (in_a || in_b) && vmax == best) {
 #pragma unroll
             for (int n1 = 0; n1 < R; ++n1) {
-                const uint32_t i0 = (uint32_t)(2 * q) * M + L * n1 + lane;
-                if (x[n1].re == best) bi = min(bi, i0);
-                if (x[n1].im == best) bi = min(bi, i0 + M);
+                const int xx = L * n1 + lane;
+                if (xx <= s || xx > s + pad) {
+                    const uint32_t rx = (uint32_t)(xx <= s ? xx : xx - pad);
+                    if (in_a && x[n1].re == best) bi = min(bi, ra + rx);
+                    if (in_b && x[n1].im == best) bi = min(bi, rb + rx);
+                }
             }
         }
         bi = block_min_u32<NT>(bi, si);
 
         // ---- D. gather (FFT position policy, clamp to [0, 2s-1]) and fit
         const int fit = a.fit, kk = fit * fit, r = (fit - 1) / 2;
-        const size2 cross{(uint32_t)M - 1, (uint32_t)M - 1};
+        const size2 cross{s2 - 1, s2 - 1};
         const size2 mp = shift_pos(POLICY_FFT, bi, cross);
         for (int t = tid; t < kk; t += NT) {
             const int i = t % fit, j = t / fit;
             const size2 cc = clamp_pos((int)mp.x - r + i, (int)mp.y - r + j, cross);
-            const int qx = (int)((cc.x + S + 1) % M), qy = (int)((cc.y + S + 1) % M);
+            int qx = (int)((cc.x + s + 1) % s2), qy = (int)((cc.y + s + 1) % s2);
+            if (qx > s) qx += pad;
+            if (qy > s) qy += pad;
             const C z = B[(qy >> 1) * CS + qx];
             const T v = (qy & 1) ? z.im : z.re;
             nbv[t] = v;
@@ -344,6 +374,12 @@ constexpr size_t kMaxLds = 160 * 1024;
 
 int fft_max_side(bool is_double) { return is_double ? 128 : 256; }
 
+uint32_t fft_padded(uint32_t side) {
+    uint32_t m = 4;
+    while (m < 2 * side) m *= 2;
+    return m;
+}
+
 template <class IN, class SUM, bool FP, bool PIPE>
 void launch_fft256(fft_args<float, IN, SUM> a, hipStream_t st);
 
@@ -360,8 +396,10 @@ void launch_fft_m(const fft_args<T, IN, SUM>& a, uint32_t grid, hipStream_t st) 
 }
 
 template <class T, class IN, class SUM, bool FP, bool PIPE>
-void launch_fft_side(uint32_t side, const fft_args<T, IN, SUM>& a, uint32_t grid, hipStream_t st) {
-    switch (2 * side) {
+void launch_fft_side(uint32_t side, fft_args<T, IN, SUM> a, uint32_t grid, hipStream_t st) {
+    if (side < 2) throw unsupported_error("FFT policy: slice side must be at least 2");
+    a.side = side;
+    switch (fft_padded(side)) {
         case 4: return launch_fft_m<T, IN, SUM, 4, FP, PIPE>(a, grid, st);
         case 8: return launch_fft_m<T, IN, SUM, 8, FP, PIPE>(a, grid, st);
         case 16: return launch_fft_m<T, IN, SUM, 16, FP, PIPE>(a, grid, st);
@@ -370,9 +408,9 @@ void launch_fft_side(uint32_t side, const fft_args<T, IN, SUM>& a, uint32_t grid
         case 128: return launch_fft_m<T, IN, SUM, 128, FP, PIPE>(a, grid, st);
         case 256:
             if constexpr (std::is_same<T, float>::value) return launch_fft256<IN, SUM, FP, PIPE>(a, st);
-            throw unsupported_error("FFT policy: slice side 128 needs T = float");
+            throw unsupported_error("FFT policy: slice sides above 64 need T = float");
         default:
-            throw unsupported_error("FFT policy: slice side must be a power of two between 2 and 128 (got " +
+            throw unsupported_error("FFT policy: slice side must be between 2 and 128 (got " +
                                     std::to_string(side) + ")");
     }
 }
@@ -425,7 +463,7 @@ template <class T>
 void launch_fft_corr_prepared(const T* pics, const T* ref, T* res, uint32_t side, uint32_t ref_slices,
                               uint32_t batch, hipStream_t st) {
     if (ref_slices == 0 || batch == 0) return;
-    const size_t nc = side + 1, m = 2 * (size_t)side;
+    const size_t m = fft_padded(side), nc = m / 2 + 1;
     cplx<T>* spectra = nullptr;
     EMIDA_HIP(hipMallocAsync((void**)&spectra, nc * m * ref_slices * sizeof(cplx<T>), st));
     fft_args<T, T, T> a{};
@@ -442,7 +480,7 @@ void launch_fft_corr_prepared(const T* pics, const T* ref, T* res, uint32_t side
     b.fit = 3;
     b.lsq = lstsq_device_table();
     b.corr_out = res;
-    if (side > 64) EMIDA_HIP(hipMallocAsync((void**)&b.work, fft256_work_bytes(b.nb, b.total), st));
+    if (m > 128) EMIDA_HIP(hipMallocAsync((void**)&b.work, fft256_work_bytes(b.nb, b.total), st));
     launch_fft_side<T, T, T, false, true>(side, b, b.total, st);
     if (b.work) EMIDA_HIP(hipFreeAsync(b.work, st));
     EMIDA_HIP(hipFreeAsync(spectra, st));

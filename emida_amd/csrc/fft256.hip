@@ -1,11 +1,14 @@
-// FFT cross-correlation for 128 x 128 slices (256 x 256 transforms, float) in two
-// kernels.  The half spectrum of one slice (129 x 256 complex) does not fit in LDS,
+// FFT cross-correlation for slices of side 65..128 (256 x 256 transforms, float) in
+// two kernels.  The half spectrum of one slice (129 x 256 complex) does not fit in LDS,
 // but the column stage needs one column at a time: kernel 1 keeps the 64 packed
 // row-pair spectra in LDS, runs every column's forward transform, product with the
 // conjugate reference spectrum and inverse transform, and writes the column results
 // (129 x 256 complex per slice) to a work buffer; kernel 2 reads them back in bands
 // of 32 rows, runs the packed row-pair inverse transforms, finds the arg-max over the
 // 256 x 256 circular buffer, recomputes the F rows of the neighbourhood, and fits.
+// A slice side s < 128 is zero-padded to 256; the arg-max, the gather and the
+// optional result buffer use the reference's 2s x 2s layout, as in fft.hip (index
+// k <= s is circular position k, k > s is position k + 256 - 2s).
 //
 // Same semantics and scaling as the single-kernel pipeline (fft.hip): reference
 // src/gpu_offset.hpp:303-316, src/hadamard.cu:27-41, src/maxarg.cu,
@@ -19,7 +22,7 @@ namespace emida_amd {
 
 namespace {
 
-constexpr int kM = 256, kS = 128, kNC = kS + 1, kL = 16, kR = 16;
+constexpr int kM = 256, kS = 128, kNC = kS + 1, kL = 16, kR = 16;  // kS: largest slice side
 constexpr int kCS = 280;          // LDS slot stride (complex): >= L (R + 1), 8 mod 16
 constexpr int kNT1 = 1024;        // kernel 1: 64 lane groups, one per row pair / two column jobs
 constexpr int kNT2 = 256;         // kernel 2: 16 lane groups, one per row pair of a band
@@ -52,6 +55,9 @@ __global__ __launch_bounds__(kNT1, 1) void fft256_cols_kernel(fft_args<float, IN
     const uint32_t sn = a.first + sl;
     const uint32_t bn = sn % a.nb;
     const int tid = threadIdx.x, lane = tid % kL, grp = tid / kL;
+    const int s = (int)a.side;  // slice side, s <= kS
+    // stored reference spectra: x 1/4 x (2s/256)^2 (the reference's (2s)^2 scale)
+    const float sc = (float)(0.25 * ((double)(2 * s) / kM) * ((double)(2 * s) / kM));
     init_twiddles(tw, tid, kNT1);
 
     // ---- 0. mean of the ROI (FROM_PATTERN), read again by the row stage
@@ -63,7 +69,7 @@ __global__ __launch_bounds__(kNT1, 1) void fft256_cols_kernel(fft_args<float, IN
         pic = a.pics + (size_t)(sn / a.nb) * a.src.area() + (size_t)b0.y * a.src.x + b0.x;
         if constexpr (std::is_same<IN, uint16_t>::value) {
             uint32_t su = 0;
-            for (int e = tid; e < kS * kS; e += kNT1) su += pic[(size_t)(e / kS) * a.src.x + (e % kS)];
+            for (int e = tid; e < s * s; e += kNT1) su += pic[(size_t)(e / s) * a.src.x + (e % s)];
 #pragma unroll
             for (int off = kWave / 2; off > 0; off >>= 1) su += __shfl_xor(su, off);
             if (tid % kWave == 0) red_u[tid / kWave] = su;
@@ -71,30 +77,34 @@ __global__ __launch_bounds__(kNT1, 1) void fft256_cols_kernel(fft_args<float, IN
             su = 0;
 #pragma unroll
             for (int k = 0; k < kNT1 / kWave; ++k) su += red_u[k];
-            mean = mean_of<float>((float)su, kS * kS);
+            mean = mean_of<float>((float)su, s * s);
         } else {
-            mean = mean_of<float>((float)a.sums[sn], kS * kS);
+            mean = mean_of<float>((float)a.sums[sn], s * s);
         }
     } else {
-        pre = a.prepared + (size_t)sn * kS * kS;
+        pre = a.prepared + (size_t)sn * s * s;
     }
     __syncthreads();
 
-    // ---- A. packed row pairs (2g, 2g + 1), forward
-    {
+    // ---- A. packed row pairs (2g, 2g + 1), forward (an odd s leaves the last pair's
+    // second row zero)
+    if (grp < (s + 1) / 2) {
         const int g = grp;
+        const bool second = 2 * g + 1 < s;
         C x[kR];
 #pragma unroll
         for (int n1 = 0; n1 < kR; ++n1) {
-            if (n1 < kR / 2) {
-                const int n = kL * n1 + lane;
-                float v0, v1;
+            const int n = kL * n1 + lane;
+            if (n1 < kR / 2 && n < s) {
+                float v0, v1 = 0.f;
                 if constexpr (FROM_PATTERN) {
                     v0 = prep_px<float, IN>(pic[(size_t)(2 * g) * a.src.x + n], mean, a.hx[n], a.hy[2 * g]);
-                    v1 = prep_px<float, IN>(pic[(size_t)(2 * g + 1) * a.src.x + n], mean, a.hx[n], a.hy[2 * g + 1]);
+                    if (second)
+                        v1 = prep_px<float, IN>(pic[(size_t)(2 * g + 1) * a.src.x + n], mean, a.hx[n],
+                                                a.hy[2 * g + 1]);
                 } else {
-                    v0 = pre[(2 * g) * kS + n];
-                    v1 = pre[(2 * g + 1) * kS + n];
+                    v0 = pre[(2 * g) * s + n];
+                    if (second) v1 = pre[(2 * g + 1) * s + n];
                 }
                 x[n1] = C{v0, v1};
             } else {
@@ -115,8 +125,8 @@ __global__ __launch_bounds__(kNT1, 1) void fft256_cols_kernel(fft_args<float, IN
         const int c = grp + 64 * j;
 #pragma unroll
         for (int n1 = 0; n1 < kR; ++n1) {
-            if (n1 < kR / 2) {
-                const int y = kL * n1 + lane;
+            const int y = kL * n1 + lane;
+            if (n1 < kR / 2 && y < s) {
                 const C* row = B + (y >> 1) * kCS;
                 if (c == 0) {
                     const C z0 = row[0], zs = row[kS];
@@ -167,8 +177,8 @@ __global__ __launch_bounds__(kNT1, 1) void fft256_cols_kernel(fft_args<float, IN
                 C* os = a.spectra_out + ((size_t)bn * kNC + kS) * kM + lane * kR;
 #pragma unroll
                 for (int r = 0; r < kR; ++r) {
-                    o0[r] = C{p0[r].re * 0.25f, p0[r].im * 0.25f};
-                    os[r] = C{ps[r].re * 0.25f, ps[r].im * 0.25f};
+                    o0[r] = C{p0[r].re * sc, p0[r].im * sc};
+                    os[r] = C{ps[r].re * sc, ps[r].im * sc};
                 }
             }
         } else {
@@ -179,7 +189,7 @@ __global__ __launch_bounds__(kNT1, 1) void fft256_cols_kernel(fft_args<float, IN
             } else {
                 C* oc = a.spectra_out + ((size_t)bn * kNC + c) * kM + lane * kR;
 #pragma unroll
-                for (int r = 0; r < kR; ++r) oc[r] = C{x[r].re * 0.25f, x[r].im * 0.25f};
+                for (int r = 0; r < kR; ++r) oc[r] = C{x[r].re * sc, x[r].im * sc};
             }
         }
         if constexpr (PIPE) {
@@ -249,13 +259,16 @@ __global__ __launch_bounds__(kNT2, 2) void fft256_rows_kernel(fft_args<float, IN
     const C* g = a.work + (size_t)sl * kNC * kM;
     C* myscr = scr + grp * kCS;
     init_twiddles(tw, tid, kNT2);
-    float* cout = a.corr_out ? a.corr_out + (size_t)sn * kM * kM : nullptr;
+    const int s = (int)a.side, pad = kM - 2 * s;
+    const uint32_t s2 = 2 * s;
+    float* cout = a.corr_out ? a.corr_out + (size_t)sn * s2 * s2 : nullptr;
 
     float vbest = -(float)__builtin_huge_val();
     uint32_t ibest = 0xffffffffu;
     C x[kR];  // one array for both inverse stages (keeps it in registers)
 #pragma unroll 1
     for (int y0 = 0; y0 < kM; y0 += kBand) {
+        if (y0 > s && y0 + kBand - 1 <= s + pad) continue;  // no row of the reference layout
         __syncthreads();  // previous band consumed
         for (int e = tid; e < kNC * kBand; e += kNT2) {
             const int c = e / kBand, r = e % kBand;
@@ -264,24 +277,27 @@ __global__ __launch_bounds__(kNT2, 2) void fft256_rows_kernel(fft_args<float, IN
         __syncthreads();
         const int p = grp;  // row pair within the band
         row_pair_inverse(x, lane, myscr, tw, Gb + 2 * p, kBS);
-        const uint32_t row0 = (uint32_t)(y0 + 2 * p);
+        const int ya = y0 + 2 * p, yb = ya + 1;
+        const bool in_a = ya <= s || ya > s + pad, in_b = yb <= s || yb > s + pad;
+        const uint32_t ra = (uint32_t)(ya <= s ? ya : ya - pad) * s2, rb_ = (uint32_t)(yb <= s ? yb : yb - pad) * s2;
 #pragma unroll
         for (int n1 = 0; n1 < kR; ++n1) {
-            const uint32_t col = kL * n1 + lane;
-            const uint32_t i0 = row0 * kM + col;
+            const int col = kL * n1 + lane;
+            if (col > s && col <= s + pad) continue;
+            const uint32_t rx = (uint32_t)(col <= s ? col : col - pad);
             // strictly larger wins; equal values keep the lower index (the scan order
             // per thread is increasing in neither, so compare indices)
-            if (x[n1].re > vbest || (x[n1].re == vbest && i0 < ibest)) {
+            if (in_a && (x[n1].re > vbest || (x[n1].re == vbest && ra + rx < ibest))) {
                 vbest = x[n1].re;
-                ibest = i0;
+                ibest = ra + rx;
             }
-            if (x[n1].im > vbest || (x[n1].im == vbest && i0 + kM < ibest)) {
+            if (in_b && (x[n1].im > vbest || (x[n1].im == vbest && rb_ + rx < ibest))) {
                 vbest = x[n1].im;
-                ibest = i0 + kM;
+                ibest = rb_ + rx;
             }
             if (cout) {
-                cout[i0] = x[n1].re;
-                cout[i0 + kM] = x[n1].im;
+                if (in_a) cout[ra + rx] = x[n1].re;
+                if (in_b) cout[rb_ + rx] = x[n1].im;
             }
         }
     }
@@ -292,12 +308,13 @@ __global__ __launch_bounds__(kNT2, 2) void fft256_rows_kernel(fft_args<float, IN
 
     // ---- neighbourhood: recompute the F rows it touches (one lane group per row)
     const int fit = a.fit, kk = fit * fit, r = (fit - 1) / 2;
-    const size2 cross{(uint32_t)kM - 1, (uint32_t)kM - 1};
+    const size2 cross{s2 - 1, s2 - 1};
     const size2 mp = shift_pos(POLICY_FFT, bi == 0xffffffffu ? 0u : bi, cross);
     __syncthreads();  // band buffer free: rb reuses it
     if (grp < fit) {
         const size2 cc = clamp_pos((int)mp.x, (int)mp.y - r + grp, cross);
-        const int qy = (int)((cc.y + kS + 1) % kM);
+        int qy = (int)((cc.y + s + 1) % s2);
+        if (qy > s) qy += pad;
         const int p2 = qy >> 1;
         row_pair_inverse(x, lane, myscr, tw, g + 2 * p2, kM);
         // branch outside the loop: a select between the two members would index the
@@ -314,7 +331,8 @@ __global__ __launch_bounds__(kNT2, 2) void fft256_rows_kernel(fft_args<float, IN
     for (int t = tid; t < kk; t += kNT2) {
         const int i = t % fit, j = t / fit;
         const size2 cc = clamp_pos((int)mp.x - r + i, (int)mp.y - r + j, cross);
-        const int qx = (int)((cc.x + kS + 1) % kM);
+        int qx = (int)((cc.x + s + 1) % s2);
+        if (qx > s) qx += pad;
         const float v = rb[j * kM + qx];
         nbv[t] = v;
         if (a.neigh) a.neigh[(size_t)sn * kk + t] = v;

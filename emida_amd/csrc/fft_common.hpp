@@ -177,6 +177,7 @@ struct fft_args {
     const T* hy;
     const size2* begins;     // ROI corners
     size2 src;               // pattern size
+    uint32_t side;           // slice side s (at most half the transform size)
     uint32_t nb;             // ROIs per pattern
     uint32_t total;          // slices in this launch
     const T* prepared;       // prepared s x s slices (!FROM_PATTERN)
@@ -188,7 +189,7 @@ struct fft_args {
     T* neigh;
     double* offsets;
     double* coefs;
-    T* corr_out;             // optional (2s)^2 circular buffers, row-major
+    T* corr_out;             // optional (2s)^2 circular buffers in the reference layout, row-major
     cplx<T>* work;           // two-kernel path: column-stage results of this launch's slices
     uint32_t first;          // two-kernel path: first slice of this launch (a multiple of nb)
 };

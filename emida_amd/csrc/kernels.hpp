@@ -60,6 +60,8 @@ void mfma_refine_bounds(int mode, size2 s, bool t_double, double& tol_rel, doubl
 // Largest padded FFT side: 128 for double (single-kernel pipeline), 256 for float
 // (M = 256 runs as two kernels with a work buffer, see fft256.hip).
 int fft_max_side(bool is_double);
+// Transform side M of a slice side s: the power of two >= 2s (at least 4).
+uint32_t fft_padded(uint32_t side);
 // M = 256: slices per launch chunk and the work buffer a launch of `total` slices needs.
 constexpr uint32_t kFft256Chunk = 4096;
 size_t fft256_chunk(uint32_t nb);
@@ -71,7 +73,7 @@ void launch_fft_ref(const IN* pics, const S* sums, const T* hx, const T* hy, con
                     uint32_t side, uint32_t nb, cplx<T>* spectra, hipStream_t st);
 // Whole FFT pipeline of `total` slices (patterns x ROIs): prepare, R2C, multiply by
 // conj(reference spectrum), C2R, arg-max, gather, fit.  corr_out (optional)
-// receives the (2s)^2 circular correlation buffers; work (side 128 only) holds
+// receives the (2s)^2 circular correlation buffers; work (M = 256 only) holds
 // fft256_work_bytes(nb, total) bytes.
 template <class T, class IN, class S>
 void launch_fft_pipe(const IN* pics, const S* sums, const T* hx, const T* hy, const size2* begins, size2 src,

@@ -212,7 +212,7 @@ struct emida_offset_s {
             parts[2] = al256(tn * cross.area() * sizeof(float));
             parts[3] = al256(tn * slice.area() * 2);
             if (mfma_split(mfma_mode)) parts[4] = parts[3];
-        } else if (side > 64) {
+        } else if (fft_padded(side) > 128) {
             parts[2] = al256(fft256_work_bytes(nb, (uint32_t)tn));  // column-stage work buffer
         }
         return parts[0] + parts[1] + parts[2] + parts[3] + parts[4];
@@ -281,6 +281,38 @@ struct emida_offset_s {
 
     void require_allocated() const {
         if (!allocated) throw state_error("allocate_memory must be called first");
+    }
+
+    // Upload one batch of pictures into the next input buffer (round robin), after
+    // the kernels that read its previous contents finished and get_offset_core
+    // consumed it; `staged`, when given, is recorded once the host buffer was read.
+    uint32_t upload(const void* pic, hipEvent_t staged = nullptr) {
+        require_allocated();
+        std::unique_lock<std::mutex> lock(upload_mu);
+        const uint32_t i = rr.fetch_add(1) % n_in;
+        upload_cv.wait(lock, [&] { return !pending[i]; });
+        pending[i] = 1;
+        try {
+            EMIDA_HIP(hipStreamWaitEvent(in_stream, ev_done[i], 0));
+            EMIDA_HIP(hipMemcpyAsync(d_in[i], pic, (size_t)src.area() * batch * in_bytes(), hipMemcpyHostToDevice,
+                                     in_stream));
+            EMIDA_HIP(hipEventRecord(ev_up[i], in_stream));
+            if (staged) EMIDA_HIP(hipEventRecord(staged, in_stream));
+        } catch (...) {
+            pending[i] = 0;
+            upload_cv.notify_all();
+            throw;
+        }
+        return i;
+    }
+
+    // Give an uploaded buffer back without computing on it (error paths).
+    void release(uint32_t i) {
+        {
+            std::lock_guard<std::mutex> lock(upload_mu);
+            if (i < n_in) pending[i] = 0;
+        }
+        upload_cv.notify_all();
     }
 
     // ---- reference preparation (src/gpu_offset.hpp:185-210)
@@ -376,9 +408,10 @@ struct emida_offset_s {
                 d_corr = alloc(scratch_total * cross.area() * sizeof(float));
             }
         } else {
-            const size_t spec = (size_t)(side + 1) * (2 * side) * 2 * t_bytes();
+            const size_t m = fft_padded(side);
+            const size_t spec = (m / 2 + 1) * m * 2 * t_bytes();
             d_ref = alloc(spec * nb);
-            if (side > 64 && !resident_only) d_corr = alloc(fft256_work_bytes(nb, total));
+            if (m > 128 && !resident_only) d_corr = alloc(fft256_work_bytes(nb, total));
         }
         const size_t nk = (size_t)fit * fit;
         for (auto& o : out) {
@@ -490,6 +523,19 @@ struct emida_offset_s {
         lru->maxes = maxes;
         lru->neigh = neigh;
         return *lru;
+    }
+
+    // Drop the result slot kept for these host buffers before they are freed, so a
+    // later buffer at the same address is not mistaken for them by finalize().
+    void forget_slot(const void* maxes, const void* neigh) {
+        std::lock_guard<std::mutex> lock(slot_mu);
+        for (auto& s : slots)
+            if (s.maxes == maxes && s.neigh == neigh && s.stamp != 0) {
+                EMIDA_HIP(hipEventSynchronize(s.done));
+                s.maxes = nullptr;
+                s.neigh = nullptr;
+                s.stamp = 0;
+            }
     }
 
     void core(void* maxes_i, void* neighbors, uint32_t bi) {

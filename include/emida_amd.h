@@ -209,8 +209,8 @@ int emida_run_cross_corr(emida_type T, const void* pics, const void* ref, void* 
 
 /* cross_corr_fft(), src/gpu_offset.hpp:303-316 (R2C, hadamard, C2R; unnormalised):
  * prepared size.x x size.y slices -> (2 size.x) x (2 size.y) circular correlation
- * buffers, batch x ref_slices of them.  size must be square with a power-of-two
- * side <= 64. */
+ * buffers, batch x ref_slices of them.  size must be square with a side of 2..128
+ * (F32) or 2..64 (F64). */
 int emida_run_cross_corr_fft(emida_type T, const void* pics, const void* ref, void* res, emida_size2 size,
                              uint32_t ref_slices, uint32_t batch_size, void* stream);
 
@@ -248,9 +248,9 @@ int emida_run_extract_neighbors(emida_type T, emida_cross_policy pos_policy, con
 int emida_run_subpixel_max(emida_type T, const void* neighbors, int s, uint32_t n, double* offsets,
                            double* coefs, void* stream);
 
-/* Largest slice side the FFT policy accepts for element type T (slice sides are
- * powers of two up to this).  No counterpart in the reference, where cuFFT plans any
- * size. */
+/* Largest slice side the FFT policy accepts for element type T (any side from 2 up
+ * to this; the transform is padded to the next power of two >= 2s, with the same
+ * correlation values).  No counterpart in the reference, where cuFFT plans any size. */
 int emida_fft_max_slice(emida_type T, uint32_t* side);
 
 /* get_lstsq_matrix(s), src/subpixel_max.cpp:5-20: (A^T A)^-1 A^T as 6 x s^2 doubles. */
@@ -317,6 +317,69 @@ int emida_server_submit(emida_server* s, uint32_t plan_id, const uint16_t* patte
 /* Blocks until the request completed; info may be null.  A request is waited for once. */
 int emida_server_wait(emida_server* s, uint64_t request_id, emida_request_info* info);
 int emida_server_get_stats(emida_server* s, emida_server_stats* stats);
+
+/* ======================================================================
+ * Input stage: raw-strip TIFF patterns and the multi-threaded file pipeline.
+ * ====================================================================== */
+
+/* get_size(fname), src/load_tiff.hpp:57-69: width (x) and height (y) of the first
+ * image.  EMIDA_ERR_ARG when the file cannot be opened or is not a classic TIFF. */
+int emida_tiff_size(const char* path, emida_size2* size);
+
+/* load_tiff(file_name, raster, size), src/load_tiff.hpp:14-53: the strips of an
+ * uncompressed 16-bit single-sample TIFF copied in file order into raster
+ * (size.x * size.y uint16, row-major, no byte swapping).  EMIDA_ERR_ARG for a size
+ * other than `size`, an orientation other than top-left, or strips that are missing,
+ * truncated or larger than the raster; EMIDA_ERR_UNSUPPORTED for compressed images
+ * and other pixel formats (the reference copies their raw bytes regardless). */
+int emida_tiff_load(const char* path, uint16_t* raster, emida_size2 size);
+
+/* Results of one batch of files: files first .. first + count - 1 of the list;
+ * offsets (double[2 * count * n_begins]) and coefs (double[6 * count * n_begins])
+ * in file-major order, valid during the call.  status != EMIDA_OK (offsets and
+ * coefs null, error names the file) when a file of the batch could not be read or
+ * the pipeline stopped. */
+typedef void (*emida_files_callback)(void* user, uint32_t first, uint32_t count, int status, const char* error,
+                                     const double* offsets, const double* coefs);
+
+typedef struct {
+    uint32_t batches;        /* batches of batch_size files */
+    uint32_t failed_batches; /* batches reported with status != EMIDA_OK */
+    double wall_ms;          /* the whole call */
+    double load_ms;          /* summed over loader threads: waiting for a free host buffer and reading files */
+    double upload_wait_ms;   /* summed over loader threads: waiting for a free device input buffer */
+} emida_files_stats;
+
+/* file_processor::process_files, src/process_files.hpp:135-241, for an allocated
+ * emida_offset with IN = U16: load_workers threads (the caller's included) read
+ * batch_size files each into pinned buffers and upload them; one thread runs
+ * get_offset_core per batch; one thread finalizes and calls cb once per batch (in
+ * file order when `ordered`, else in completion order).  A file that cannot be read
+ * fails its batch only.  Returns EMIDA_OK unless the pipeline itself failed (a HIP
+ * error); EMIDA_ERR_ARG for an empty list.  The handle must not be used by other
+ * threads during the call; the emida_offset's in_buffers_size sets how many uploads
+ * may be in flight (the reference uses 2 * load_workers). */
+int emida_process_files(emida_offset* h, const char* const* paths, uint32_t n_paths, uint32_t load_workers,
+                        int ordered, emida_files_callback cb, void* user, emida_files_stats* stats);
+
+/* load_slice_begins(file_name), src/args_parser.cpp:62-108: the slicepos file.  Its
+ * first line holds the ROI "radius" (half the slice size; *radius_ok = 0 when it does
+ * not parse), each further line "<mid_x> <mid_y>" (std::stoul rules: leading digits,
+ * trailing characters ignored).  Lines without a space or with an unparsable number
+ * are skipped with a warning on stderr (printed by calls that pass `mids`, so a
+ * counting call with mids = NULL stays quiet).  Up to `capacity` middles are written;
+ * *count receives the number of middles in the file. */
+int emida_read_slicepos(const char* path, uint64_t* radius, int* radius_ok, emida_size2* mids, uint32_t capacity,
+                        uint32_t* count);
+
+/* file_processor::load_work, src/process_files.hpp:54-73: the deformed list, one
+ * pattern per line, "<x> <y> <file name>" (the name is the rest of the line after the
+ * character that follows y).  Up to `capacity` (x, y) pairs go to xy; the names, each
+ * NUL-terminated, are packed into `names` while they fit in names_capacity bytes;
+ * *count and *names_bytes receive the totals.  A line whose x or y does not parse
+ * gives x = y = 0 and an empty name. */
+int emida_read_deformed_list(const char* path, double* xy, uint32_t capacity, uint32_t* count, char* names,
+                             size_t names_capacity, size_t* names_bytes);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,12 @@ inputs are data files that the reference's tests read:
   test/res/cross_corr/64_{A,B,res}.txt   64x64 ints, 127x127 int result
   test/res/hann/{5,128}_{A,res}.txt      window fixtures (numpy.hanning applied)
 
+and copies, byte for byte, the data files of the file driver's tests and example run:
+
+  test/res/INITIAL_x0y0.tif              873x873 uint16 pattern (load_tiff test)
+  TEST_RUN/roi-cryst.txt                 slicepos file: radius 48, 89 ROI middles
+  TEST_RUN/def.txt                       deformed list: 492 "<x> <y> <file>" lines
+
 Text values are parsed with Python's correctly rounded float(), so the .npy files
 hold exactly the doubles the reference's matrix::from_file reads
 (src/matrix.hpp:20-32). SHA-256 of every output is written to fixtures.sha256.
@@ -54,6 +60,17 @@ def main():
         np.save(path, arr, allow_pickle=False)
         digest = hashlib.sha256(open(path, "rb").read()).hexdigest()
         lines.append(f"{digest}  {dst}  (from {src}, shape {arr.shape}, {arr.dtype})")
+        print(lines[-1])
+    copies = [
+        ("test/res/INITIAL_x0y0.tif", "INITIAL_x0y0.tif"),
+        ("TEST_RUN/roi-cryst.txt", "roi-cryst.txt"),
+        ("TEST_RUN/def.txt", "def.txt"),
+    ]
+    for src, dst in copies:
+        data = open(os.path.join(REF, src), "rb").read()
+        with open(os.path.join(OUT, dst), "wb") as f:
+            f.write(data)
+        lines.append(f"{hashlib.sha256(data).hexdigest()}  {dst}  (copy of {src}, {len(data)} bytes)")
         print(lines[-1])
     with open(os.path.join(OUT, "fixtures.sha256"), "w") as f:
         f.write("\n".join(lines) + "\n")

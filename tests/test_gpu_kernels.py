@@ -198,9 +198,10 @@ def test_cross_corr_int32_wraps_like_oracle():
 
 
 # ------------------------------------------------------------------ FFT correlation
-@pytest.mark.parametrize("s", [2, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("s", [2, 4, 8, 16, 32, 64, 3, 5, 6, 20, 33, 48, 63])
 @pytest.mark.parametrize("dt,tol", [(np.float64, 1e-12), (np.float32, 2e-6)])
 def test_cross_corr_fft_vs_oracle(s, dt, tol):
+    # sides that are not powers of two: padded transforms, reference (2s)^2 layout
     rng = np.random.default_rng(14 + s)
     ref_slices, batch = 3, 2
     pics = rng.standard_normal(s * s * ref_slices * batch).astype(dt)
@@ -212,10 +213,9 @@ def test_cross_corr_fft_vs_oracle(s, dt, tol):
     assert (np.abs(got - exp) / scale).max() < tol * max(1, np.log2(2 * s))
 
 
-@pytest.mark.parametrize("ref_slices,batch", [(2, 3), (1, 1)])
-def test_cross_corr_fft_128_float_vs_oracle(ref_slices, batch):
-    # 128 x 128 slices (256-point transforms) run as two kernels with a work buffer
-    s = 128
+@pytest.mark.parametrize("s,ref_slices,batch", [(128, 2, 3), (128, 1, 1), (96, 2, 2), (65, 1, 2), (127, 1, 1)])
+def test_cross_corr_fft_128_float_vs_oracle(s, ref_slices, batch):
+    # slices of side 65..128 (256-point transforms) run as two kernels with a work buffer
     rng = np.random.default_rng(99 + batch)
     pics = rng.standard_normal(s * s * ref_slices * batch).astype(np.float32)
     ref = rng.standard_normal(s * s * ref_slices).astype(np.float32)

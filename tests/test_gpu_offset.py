@@ -127,23 +127,29 @@ def test_brute_cropped_cross_with_edge_peaks_vs_oracle(cross):
     np.testing.assert_array_equal(offs, eo)
 
 
-@pytest.mark.parametrize("fit", [3, 5, 9])
-def test_fft_double_vs_oracle(fit):
+@pytest.mark.parametrize("s,fit", [(64, 3), (64, 5), (64, 9), (48, 5), (33, 3), (20, 7)])
+def test_fft_double_vs_oracle(s, fit):
+    # sides that are not powers of two run zero-padded to the next power of two
     ps = _patterns(256, 2)
-    begins = synth.grid_begins(256, 4, 32)
-    offs, coefs, maxes, neigh = _run(np.float64, FFT, ps, begins, (64, 64), (127, 127), fit, 2)
-    eo, ec, em, en = O.get_offset(ps.pics.ravel(), ps.ref, ps.size, begins, (64, 64), (127, 127), 2, fit, FFT,
+    begins = synth.grid_begins(256, 4, (s + 1) // 2)
+    c = (2 * s - 1, 2 * s - 1)
+    offs, coefs, maxes, neigh = _run(np.float64, FFT, ps, begins, (s, s), c, fit, 2)
+    eo, ec, em, en = O.get_offset(ps.pics.ravel(), ps.ref, ps.size, begins, (s, s), c, 2, fit, FFT,
                                   np.float64, return_intermediates=True)
-    ok = _peak_margin_ok(FFT, ps, begins, (64, 64), 2, 1e-11)
+    ok = _peak_margin_ok(FFT, ps, begins, (s, s), 2, 1e-11)
     assert ok.mean() > 0.9
     np.testing.assert_array_equal(maxes["index"][ok], em["index"][ok])
     assert np.abs(offs - eo)[ok].max() < 1e-9
+    # values carry the reference's (2s)^2 scale also when the transform is padded
+    np.testing.assert_allclose(maxes["data"][ok], em["data"][ok], rtol=1e-9)
+    assert (np.abs(coefs - ec)[ok].max(axis=1) <= 1e-9 * np.abs(ec)[ok].max(axis=1)).all()
 
 
-@pytest.mark.parametrize("s,fit", [(32, 5), (16, 3), (8, 3), (64, 7), (128, 7), (128, 9)])
+@pytest.mark.parametrize("s,fit", [(32, 5), (16, 3), (8, 3), (64, 7), (128, 7), (128, 9), (48, 5), (21, 3),
+                                   (96, 7), (100, 5), (127, 9), (65, 3)])
 def test_fft_float_vs_oracle(s, fit):
     ps = _patterns(256, 2)
-    begins = synth.grid_begins(256, 5, s // 2)
+    begins = synth.grid_begins(256, 5, (s + 1) // 2)
     offs, coefs, maxes, neigh = _run(np.float32, FFT, ps, begins, (s, s), (2 * s - 1, 2 * s - 1), fit, 2)
     eo, ec, em, en = O.get_offset(ps.pics.ravel(), ps.ref, ps.size, begins, (s, s), (2 * s - 1, 2 * s - 1), 2, fit,
                                   FFT, np.float64, return_intermediates=True)

@@ -54,7 +54,7 @@ def test_create_valid_configurations_without_gpu():
     (dict(batch_size=0), C.ERR_ARG),
     (dict(begins=[]), C.ERR_ARG),
     (dict(IN=np.int32), C.ERR_ARG),
-    (dict(policy=emida_amd.CROSS_POLICY_FFT, slice_size=(48, 48)), C.ERR_UNSUPPORTED),
+    (dict(policy=emida_amd.CROSS_POLICY_FFT, slice_size=(65, 65), src_size=(300, 300)), C.ERR_UNSUPPORTED),
     (dict(policy=emida_amd.CROSS_POLICY_FFT, slice_size=(32, 16)), C.ERR_UNSUPPORTED),
     (dict(policy=emida_amd.CROSS_POLICY_FFT, slice_size=(128, 128)), C.ERR_UNSUPPORTED),  # double: <= 64
     (dict(policy=emida_amd.CROSS_POLICY_FFT, T=np.float32, slice_size=(256, 256), src_size=(300, 300)),
@@ -74,6 +74,11 @@ def test_fft_limits():
     assert emida_amd.fft_max_slice(np.float32) == 128 and emida_amd.fft_max_slice(np.float64) == 64
     g = _create(policy=emida_amd.CROSS_POLICY_FFT, T=np.float32, slice_size=(128, 128), cross_size=(1, 1))
     assert g.cross_size == (255, 255) and g.fft_size == 256
+    # any side: transforms padded to the next power of two >= 2s, reference buffer 2s
+    for T, side, m in [(np.float64, 48, 128), (np.float64, 3, 8), (np.float32, 96, 256), (np.float32, 65, 256),
+                       (np.float64, 64, 128), (np.float32, 33, 128)]:
+        g = _create(policy=emida_amd.CROSS_POLICY_FFT, T=T, slice_size=(side, side), src_size=(300, 300))
+        assert g.fft_size == m and g.corr_buffer == (2 * side, 2 * side) and g.cross_size == (2 * side - 1,) * 2
 
 
 def test_create_matrix_core_policy_without_gpu():

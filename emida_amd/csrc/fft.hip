@@ -71,7 +71,9 @@ struct fft_plan {
     static constexpr int WAVES_PER_SIMD = WAVES_PER_SIMD_RAW < 1 ? 1 : (WAVES_PER_SIMD_RAW > 4 ? 4 : WAVES_PER_SIMD_RAW);
 };
 
-template <class T, class IN, class SUM, int M, bool FROM_PATTERN, bool PIPE>
+// PAD: the slice side is below M/2 (a runtime value); otherwise it is M/2 and every
+// index expression below folds to constants.
+template <class T, class IN, class SUM, int M, bool FROM_PATTERN, bool PIPE, bool PAD>
 __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SIMD)) void fft_kernel(fft_args<T, IN, SUM> a) {
     using P = fft_plan<T, M>;
     using C = cplx<T>;
@@ -91,8 +93,8 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
     const uint32_t bn = sn % a.nb;
     const int tid = threadIdx.x;
     const int lane = tid % L, grp = tid / L;
-    const int s = (int)a.side;   // slice side, 2 <= s <= S
-    const int pad = M - 2 * s;   // reference index k > s sits at circular position k + pad
+    const int s = PAD ? (int)a.side : S;  // slice side, 2 <= s <= S
+    const int pad = M - 2 * s;            // reference index k > s sits at circular position k + pad
     // stored reference spectra: x 1/4 (the omitted split factors) x (2s/M)^2, so an
     // M-point C2R carries the reference's (2s)^2 scale; s^2 / M^2 is exact in T
     const T sc = (T)(0.25 * ((double)(2 * s) / M) * ((double)(2 * s) / M));
@@ -383,16 +385,22 @@ uint32_t fft_padded(uint32_t side) {
 template <class IN, class SUM, bool FP, bool PIPE>
 void launch_fft256(fft_args<float, IN, SUM> a, hipStream_t st);
 
-template <class T, class IN, class SUM, int M, bool FP, bool PIPE>
-void launch_fft_m(const fft_args<T, IN, SUM>& a, uint32_t grid, hipStream_t st) {
+template <class T, class IN, class SUM, int M, bool FP, bool PIPE, bool PAD>
+void launch_fft_mp(const fft_args<T, IN, SUM>& a, uint32_t grid, hipStream_t st) {
     using P = fft_plan<T, M>;
     static_assert(P::lds_bytes + 2048 <= kMaxLds, "FFT working set exceeds LDS");
-    auto kern = fft_kernel<T, IN, SUM, M, FP, PIPE>;
+    auto kern = fft_kernel<T, IN, SUM, M, FP, PIPE, PAD>;
     static const hipError_t attr =
         hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P::lds_bytes);
     EMIDA_HIP(attr);
     kern<<<grid, P::NT, P::lds_bytes, st>>>(a);
     EMIDA_LAUNCHED("fft_kernel");
+}
+
+template <class T, class IN, class SUM, int M, bool FP, bool PIPE>
+void launch_fft_m(const fft_args<T, IN, SUM>& a, uint32_t grid, hipStream_t st) {
+    if (2 * a.side == (uint32_t)M) launch_fft_mp<T, IN, SUM, M, FP, PIPE, false>(a, grid, st);
+    else launch_fft_mp<T, IN, SUM, M, FP, PIPE, true>(a, grid, st);
 }
 
 template <class T, class IN, class SUM, bool FP, bool PIPE>

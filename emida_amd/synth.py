@@ -194,4 +194,8 @@ CONFIGS = {
     # brute, fitsize 9, error vs FP64 on a subset
     "scan_1024": dict(size=1024, patterns=20000, k=31, radius=32, fit=9, subset=500, t_range=2.0, policy="scan",
                       dtype="f32", hbm_budget=64 << 30),
+    # the file driver end to end: 1024 TIFF files of 873 x 873 read by 1..16 loader
+    # threads, uploaded and run through the FFT policy (400 ROIs of 64 x 64, float)
+    "files_873": dict(size=873, patterns=1024, k=20, radius=32, fit=5, batch=8, t_range=3.0, policy="files",
+                      dtype="f32", workers=(1, 2, 4, 8, 16)),
 }

@@ -9,15 +9,16 @@
 // reference rows y, where B_y is the Toeplitz matrix of reference row y (zero outside
 // the row).  All-zero Toeplitz tiles are skipped.
 //
-// Work decomposition: one workgroup = 32 patterns x every x-shift (4 waves x two
-// adjacent 32-wide tiles) x TS consecutive y-shifts, on a 1-D grid whose order keeps
-// the y-shift tiles of one (ROI, pattern tile) on one XCD.  It walks the reference
-// rows y; the pattern rows Y = y + sy of its y-shifts live in an LDS ring (one new
-// row per step), and reference row y is staged as 8 shifted copies so that any
-// 8-element window - one lane's B fragment - is a single aligned 16-byte LDS read.
-// Per K tile a wave reads each B fragment once for all TS y-shifts and each A
-// fragment once for both of its x-shift tiles.  The next step's rows are loaded
-// into registers before the matrix-core work of the current step.
+// Work decomposition: one workgroup = 32 patterns x 64 WX x-shifts (WX waves side by
+// side, two adjacent 32-wide tiles each) x 4 (4 / WX) consecutive y-shifts (4 per
+// wave), WX = 4, 2 or 1 for pattern rows of up to 128, 64 or 32 columns, on a 1-D grid
+// whose order keeps the y-shift tiles of one (ROI, pattern tile) on one XCD.  It walks
+// the reference rows y; the pattern rows Y = y + sy of its y-shifts live in an LDS ring
+// (one new row per step), and reference row y is staged as 8 shifted copies so that
+// any 8-element window - one lane's B fragment - is a single aligned 16-byte LDS read.
+// Per K tile a wave reads each B fragment once for its 4 y-shifts and each A fragment
+// once for both of its x-shift tiles.  The next step's rows are loaded into registers
+// before the matrix-core work of the current step.
 //
 // Matrix-core operand layout (v_mfma_f32_32x32x16_{bf16,f16}): lane l holds
 // A[l % 32][8 (l / 32) + t] and B[8 (l / 32) + t][l % 32] for t < 8; the 16

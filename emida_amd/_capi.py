@@ -70,6 +70,10 @@ FILES_CALLBACK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint32, ctypes
                                   ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double))
 
 
+class Calib(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_double), ("y", ctypes.c_double), ("z", ctypes.c_double)]
+
+
 class OffsetInfo(ctypes.Structure):
     _fields_ = [("total_slices", ctypes.c_uint32), ("neigh_size", ctypes.c_uint32),
                 ("data_index_bytes", ctypes.c_uint32), ("cross_size", Size2), ("corr_buffer", Size2),
@@ -145,6 +149,8 @@ def lib():
                                      ctypes.POINTER(u32)], i32),
             "emida_read_deformed_list": ([ctypes.c_char_p, dp, u32, ctypes.POINTER(u32), ctypes.c_char_p, sz,
                                           ctypes.POINTER(sz)], i32),
+            "emida_fit_deformation": ([vp, i32, u32, vp, vp, u32, Calib, ctypes.c_double, u32, vp, vp, vp, vp, vp,
+                                       vp, vp], i32),
         }
         for name, (args, res) in sigs.items():
             f = getattr(L, name)

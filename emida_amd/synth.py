@@ -198,4 +198,7 @@ CONFIGS = {
     # threads, uploaded and run through the FFT policy (400 ROIs of 64 x 64, float)
     "files_873": dict(size=873, patterns=1024, k=20, radius=32, fit=5, batch=8, t_range=3.0, policy="files",
                       dtype="f32", workers=(1, 2, 4, 8, 16)),
+    # the deformation post-step over a scan's worth of ROI results: 20,000 patterns x
+    # 400 ROIs (20 x 20 grid on 873 x 873) of offsets and fit coefficients in HBM
+    "post_873": dict(size=873, patterns=20000, k=20, radius=32, policy="post", dtype="f64"),
 }

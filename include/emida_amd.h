@@ -381,6 +381,41 @@ int emida_read_slicepos(const char* path, uint64_t* radius, int* radius_ok, emid
 int emida_read_deformed_list(const char* path, double* xy, uint32_t capacity, uint32_t* count, char* names,
                              size_t names_capacity, size_t* names_bytes);
 
+/* ======================================================================
+ * Deformation post-step: per pattern, the projective map between the ROI middles and
+ * their displaced positions (an algebraic homography fit) and its strain and rotation.
+ * ====================================================================== */
+
+/* Pattern centre (x, y) and detector distance z, in pixels. */
+typedef struct {
+    double x, y, z;
+} emida_calib;
+
+/* The reference's post-processing of the offsets (python/process_offsets.py:31-97,
+ * python/test.py:137-198, jove/analyze.py:6-95), one workgroup per pattern.  All
+ * pointers are device pointers (any output may be NULL); stream-ordered on `stream`.
+ *   mids     double[n_rois][2] ROI middles (x, y), shared by all patterns, or
+ *            double[n_patterns][n_rois][2] when mids_per_pattern;
+ *   offsets  double[n_patterns][n_rois][2], as the pipeline returns them;
+ *   coefs    double[n_patterns][n_rois][6] fit coefficients, or NULL to use every ROI.
+ * With coefs, a ROI is used when d = a c - b^2 > det_min (a = -coef[3], b = -coef[4]/2,
+ * c = -coef[5]; the reference's threshold is 1e-6), quality = median of d over all ROIs,
+ * and a pattern is fitted when more than min_rois ROIs are used (the reference: 10).
+ * Without coefs every ROI is used (n_rois > 4) and quality is NaN.  Outputs per pattern:
+ *   mask     int32: 1 fitted, 0 not (its matrices are NaN);
+ *   H        double[9], the map divided by its [2][2] element (process_offsets.py);
+ *   F        double[9], the map divided by the cube root of its determinant;
+ *   epsilon  double[9], log of the right stretch U of F = R U (strain);
+ *   omega    double[9], log of R by Rodrigues' formula (rotation; NaN for exactly no
+ *            rotation, as the reference's 0/0).
+ * Matrices are row-major.  The second row of each ROI pair is [0, 0, 0, -ox z, -oy z,
+ * -z^2, ox ny, oy ny, z ny] as in process_offsets.py (jove/analyze.py:28 puts ox ny in
+ * the oy column).  n_rois <= 8192. */
+int emida_fit_deformation(const double* mids, int mids_per_pattern, uint32_t n_rois, const double* offsets,
+                          const double* coefs, uint32_t n_patterns, emida_calib calib, double det_min,
+                          uint32_t min_rois, int32_t* mask, double* quality, double* H, double* F,
+                          double* epsilon, double* omega, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -149,6 +149,7 @@ def lib():
                                      ctypes.POINTER(u32)], i32),
             "emida_read_deformed_list": ([ctypes.c_char_p, dp, u32, ctypes.POINTER(u32), ctypes.c_char_p, sz,
                                           ctypes.POINTER(sz)], i32),
+            "emida_draw_tiff": ([vp, S, ctypes.c_char_p, dp, ctypes.POINTER(S), u32], i32),
             "emida_fit_deformation": ([vp, i32, u32, vp, vp, u32, Calib, ctypes.c_double, u32, vp, vp, vp, vp, vp,
                                        vp, vp], i32),
         }

@@ -94,6 +94,20 @@ def write_tiff(path, raster, rows_per_strip=None, big_endian=False):
         f.write(hdr + body + extra + ifd)
 
 
+def draw_tiff(pic, path, vectors, positions):
+    """draw_tiff (src/draw_tiff.hpp:26-59): the (height, width) uint16 pattern as an
+    RGBA TIFF (grey = pixel >> 8) with each vector (dx, dy) drawn in red from its
+    (x, y) position."""
+    a = np.ascontiguousarray(pic, np.uint16)
+    v = np.ascontiguousarray(vectors, float).reshape(-1, 2)
+    pos = list(positions)
+    if len(pos) != len(v):
+        raise ValueError("one position per vector")
+    arr = (C.Size2 * max(1, len(pos)))(*[C.Size2(int(x), int(y)) for x, y in pos])
+    C.check(C.lib().emida_draw_tiff(C.ptr(a), C.Size2(a.shape[1], a.shape[0]), _path(path),
+                                    v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), arr, len(v)))
+
+
 def read_slicepos(path):
     """load_slice_begins (src/args_parser.cpp:62-108) -> (radius or None, mids as a
     list of (x, y))."""

@@ -381,6 +381,14 @@ int emida_read_slicepos(const char* path, uint64_t* radius, int* radius_ok, emid
 int emida_read_deformed_list(const char* path, double* xy, uint32_t capacity, uint32_t* count, char* names,
                              size_t names_capacity, size_t* names_bytes);
 
+/* draw_tiff(pic, size, file_name, vectors, vectors_pos), src/draw_tiff.hpp:26-59: writes
+ * the pattern (size.x * size.y uint16) as an 8-bit RGBA TIFF, grey = pixel >> 8, with
+ * each vector (double[2 * n_vectors], x then y) drawn in red as 20 points from its
+ * position.  Vectors with NaN components are skipped (the reference would convert NaN
+ * to int). */
+int emida_draw_tiff(const uint16_t* pic, emida_size2 size, const char* path, const double* vectors,
+                    const emida_size2* positions, uint32_t n_vectors);
+
 /* ======================================================================
  * Deformation post-step: per pattern, the projective map between the ROI middles and
  * their displaced positions (an algebraic homography fit) and its strain and rotation.

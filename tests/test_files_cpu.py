@@ -294,3 +294,24 @@ def test_process_files_needs_allocated_u16_handle(tmp_path):
     g = emida_amd.GpuOffset(np.float64, np.float64, (96, 96), [(0, 0)], (32, 32), (63, 63), 2)
     assert _err(F.process_files, g, ["a"]).code in (C.ERR_STATE, C.ERR_ARG)
     g.close()
+
+
+# ------------------------------------------------------------------ draw_tiff
+def test_draw_tiff_known_answers(tmp_path):
+    # src/draw_tiff.hpp:13-59: grey = pixel >> 8; a vector is 20 points at
+    # d = 0, 0.05, ..., 0.95 (double accumulation), rounded half away from zero
+    from PIL import Image
+    pic = (np.arange(20 * 12, dtype=np.uint32).reshape(12, 20) * 271 % 65536).astype(np.uint16)
+    p = str(tmp_path / "v.tif")
+    F.draw_tiff(pic, p, [(10.0, 0.0), (-10.0, 0.0), (0.0, 3.0), (np.nan, 1.0)], [(5, 5), (3, 8), (18, 0), (1, 1)])
+    img = np.array(Image.open(p).convert("RGBA"))
+    assert img.shape == (12, 20, 4)
+    red = np.zeros((12, 20), bool)
+    red[5, 5:16] = True        # (10, 0) from (5, 5): x offsets 0..10, all covered
+    red[8, 0:4] = True         # (-10, 0) from (3, 8): x = 3..0, then negative (skipped)
+    red[0:4, 18] = True        # (0, 3) from (18, 0): y offsets round(0.05 k * 3) = 0..3
+    exp = np.empty((12, 20, 4), np.uint8)
+    g = (pic >> 8).astype(np.uint8)
+    exp[..., 0], exp[..., 1], exp[..., 2], exp[..., 3] = g, g, g, 255
+    exp[red] = (255, 0, 0, 255)
+    np.testing.assert_array_equal(img, exp)

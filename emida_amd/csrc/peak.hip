@@ -300,18 +300,24 @@ __device__ inline T exact_at(size_t g, size_t corr_len, const T* p, const T* r, 
 // Exact values of a box of shifts of one slice, all threads of the workgroup
 // together: thread t < bw * bh takes position (x0 + t % bw, y0 + t / bw) of the cross
 // buffer (shift = position - (cross - 1) / 2).  Chunks of kBoxY reference rows and the
-// pattern rows they meet are staged in LDS (row stride s.x + 1); each thread
+// pattern rows they meet are staged in LDS (row stride box_stride(s.x)); each thread
 // accumulates its own shift in the brute order (y outer, x inner, fma), loading 8
 // products' operands ahead of the 8 dependent fmas, so the values equal exact_corr's.
 // LDS: rrow kBoxY rows, prow kBoxY + kBoxRows - 1 rows.
 constexpr int kBoxRows = 12;
 constexpr int kBoxY = 4;
 
+// LDS row stride (elements) of the staged rows: congruent to 9 mod 32, so the rows a
+// box reads at one step (consecutive shifts of up to 7 rows) start 18 banks apart
+// and a wave's 8-byte reads conflict at most 2-way (a stride of s.x + 1 put the rows
+// 2 banks apart: up to 7-way conflicts).
+__host__ __device__ constexpr int box_stride(int sx) { return sx + ((9 - sx % 32) + 32) % 32; }
+
 template <class T, int NT>
 __device__ inline void eval_box(const T* __restrict__ p, const T* __restrict__ r, size2 s, size2 cross, int x0,
                                 int y0, int bw, int bh, T* rrow, T* prow, T* out) {
     const int tid = threadIdx.x, S = (int)s.x, SY = (int)s.y;
-    const int PS = S + 1;  // LDS row stride: box rows of one wave fall on different banks
+    const int PS = box_stride(S);
     const int rx = (int)(cross.x - 1) / 2, ry = (int)(cross.y - 1) / 2;
     const int sy_lo = y0 - ry, sy_hi = y0 + bh - 1 - ry;  // shift range of the box rows
     const bool mine = tid < bw * bh;
@@ -373,8 +379,8 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
     __shared__ T boxv[NT];
     __shared__ int bb[4];
     extern __shared__ __align__(16) unsigned char refine_lds[];
-    T* rrow = reinterpret_cast<T*>(refine_lds);  // kBoxY rows of s.x + 1
-    T* prow = rrow + kBoxY * (s.x + 1);           // kBoxY + kBoxRows - 1 rows of s.x + 1
+    T* rrow = reinterpret_cast<T*>(refine_lds);  // kBoxY rows of box_stride(s.x)
+    T* prow = rrow + kBoxY * box_stride(s.x);     // kBoxY + kBoxRows - 1 rows of box_stride(s.x)
     const uint32_t sn = blockIdx.x;
     if (sn >= total) return;
     const int tid = threadIdx.x;
@@ -569,7 +575,7 @@ void launch_refine(const float* screen, size_t corr_len, const T* p, const T* r,
                    double* coefs, uint32_t total, hipStream_t st) {
     if (total == 0) return;
     const double* lsq = lstsq_device_table() + lstsq_offset(fit);
-    const size_t lds = (size_t)(2 * kBoxY + kBoxRows - 1) * (s.x + 1) * sizeof(T);
+    const size_t lds = (size_t)(2 * kBoxY + kBoxRows - 1) * box_stride(s.x) * sizeof(T);
     refine_kernel<T, kRefineNT><<<total, kRefineNT, lds, st>>>(screen, corr_len, p, r, s, cross, nb, tol_rel, tol_abs,
                                                           fit, lsq, maxes, neigh, offsets, coefs, total);
     EMIDA_LAUNCHED("refine_kernel");

@@ -315,3 +315,18 @@ def test_draw_tiff_known_answers(tmp_path):
     exp[..., 0], exp[..., 1], exp[..., 2], exp[..., 3] = g, g, g, 255
     exp[red] = (255, 0, 0, 255)
     np.testing.assert_array_equal(img, exp)
+
+
+def test_cpp_load_tiff_header(tmp_path):
+    # include/emida/load_tiff.hpp: the reference's signatures over the C ABI
+    from PIL import Image
+    exe = os.path.join(ROOT, "tests", "cpp", "test_tiff")
+    assert os.access(exe, os.X_OK), "tests/cpp/test_tiff is not built"
+    out = str(tmp_path / "drawn.tif")
+    r = subprocess.run([exe, INITIAL, out], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    a = np.array(Image.open(INITIAL)).astype(np.uint64).ravel()
+    w = (np.arange(a.size, dtype=np.uint64) % 9973)
+    assert f"size 873 873 sum {int(a.sum())} weighted {int((a * w).sum())}" in r.stdout
+    img = np.array(Image.open(out).convert("RGBA"))
+    assert tuple(img[100, 100]) == (255, 0, 0, 255) and img.shape == (873, 873, 4)

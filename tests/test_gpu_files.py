@@ -10,7 +10,8 @@
     (src/process_files.hpp:235-264) of those results, byte for byte;
   * the reference's own pattern (test/res/INITIAL_x0y0.tif) against itself, with
     the ROI file of the reference's example run (TEST_RUN/roi-cryst.txt): every
-    offset is zero.
+    offset is zero;
+  * --crosspolicy mfma prints exactly the brute policy's output (exact refinement).
 """
 import os
 import subprocess
@@ -172,3 +173,16 @@ def test_reference_pattern_against_itself_gives_zero_offsets(tmp_path):
         assert len(lines) == 2 * 90 and lines[0] == f"0.000000 0.000000 89 {init}"
         vals = np.array([[float(v) for v in ln.split()[2:]] for ln in lines[1:90]])
         assert np.abs(vals).max() < 1e-3, (policy, np.abs(vals).max())
+
+
+@pytest.mark.parametrize("precision", ["double", "float"])
+def test_cli_mfma_policy_prints_the_brute_output(scan, precision):
+    # the refined matrix-core policy gives the brute policy's results bit for bit, so
+    # the driver's output is identical text
+    pos, lst = _write_inputs(scan)
+    common = ["-i", scan["initial"], "-d", lst, "-b", pos, "--precision", precision, "--batchsize", "3", "-q", "-f",
+              "5"]
+    a = _run_cli(*common, "--crosspolicy", "brute")
+    b = _run_cli(*common, "--crosspolicy", "mfma")
+    assert a.returncode == 0 and b.returncode == 0, a.stdout + a.stderr + b.stdout + b.stderr
+    assert a.stdout == b.stdout and a.stdout.count("\n") == N_FILES * (1 + len(scan["mids"]))

@@ -270,7 +270,7 @@ __global__ void subpixel_kernel(const T* __restrict__ neigh, int s, uint32_t n, 
 // A slice with more than kRefineCap candidates (a flat, noise-dominated screen) keeps
 // the screen's arg-max.
 constexpr int kRefineCap = 512;
-constexpr int kRefineNT = 128;
+constexpr int kRefineNT = 256;
 
 template <class T>
 __device__ inline T exact_corr(const T* __restrict__ p, const T* __restrict__ r, size2 s, int sx, int sy) {
@@ -304,7 +304,7 @@ __device__ inline T exact_at(size_t g, size_t corr_len, const T* p, const T* r, 
 // accumulates its own shift in the brute order (y outer, x inner, fma), loading 8
 // products' operands ahead of the 8 dependent fmas, so the values equal exact_corr's.
 // LDS: rrow kBoxY rows, prow kBoxY + kBoxRows - 1 rows.
-constexpr int kBoxRows = 12;
+constexpr int kBoxRows = 16;
 constexpr int kBoxY = 4;
 
 // LDS row stride (elements) of the staged rows: congruent to 9 mod 32, so the rows a
@@ -476,15 +476,23 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
         atomicMax(&bb[3], cy);
     }
     __syncthreads();
+    const int k = fit * fit, rad = (fit - 1) / 2;
+    // One box evaluation when the candidates' box grown by the fit radius fits the
+    // workgroup: it holds every candidate and the in-window neighbourhood of any of
+    // them (the chains of one box run side by side; a second box would run after).
     const int cbw = bb[2] - bb[0] + 1, cbh = bb[3] - bb[1] + 1;
-    const bool boxed = cbh <= kBoxRows && cbw * cbh <= NT;
-    if (boxed) eval_box<T, NT>(ps, rs, s, cross, bb[0], bb[1], cbw, cbh, rrow, prow, boxv);
+    const int ex0 = max(0, bb[0] - rad), ey0 = max(0, bb[1] - rad);
+    const int ebw = min((int)cross.x - 1, bb[2] + rad) - ex0 + 1, ebh = min((int)cross.y - 1, bb[3] + rad) - ey0 + 1;
+    const bool merged = ebh <= kBoxRows && ebw * ebh <= NT;
+    const bool boxed = merged || (cbh <= kBoxRows && cbw * cbh <= NT);
+    const int bx0 = merged ? ex0 : bb[0], by0 = merged ? ey0 : bb[1], bw = merged ? ebw : cbw;
+    if (boxed) eval_box<T, NT>(ps, rs, s, cross, bx0, by0, bw, merged ? ebh : cbh, rrow, prow, boxv);
     T best = lowest<T>();
     uint32_t bi = 0xffffffffu;
     for (uint32_t c = tid; c < nc; c += NT) {
         const uint32_t i = cand[c];
         const int cx = (int)(i % cross.x), cy = (int)(i / cross.x);
-        const T v = boxed ? boxv[(cy - bb[1]) * cbw + (cx - bb[0])]
+        const T v = boxed ? boxv[(cy - by0) * bw + (cx - bx0)]
                           : exact_at<T>((size_t)sn * area + i, corr_len, p, r, s, cross, nb);
         if (v > best || (v == best && i < bi)) {
             best = v;
@@ -493,14 +501,18 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
     }
     block_argmax<T, NT>(best, bi, sv, si);
 
-    // neighbourhood, exact: the in-window part as one box, positions the clamp puts
-    // one past the window (spilling into the next row or slice) one chain each
-    const int k = fit * fit, rad = (fit - 1) / 2;
+    // neighbourhood, exact: the in-window part from the merged box or as its own box,
+    // positions the clamp puts one past the window (spilling into the next row or
+    // slice) one chain each
     const size2 mp = shift_pos(POLICY_BRUTE, bi, cross);
-    const int nx0 = max(0, (int)mp.x - rad), ny0 = max(0, (int)mp.y - rad);
-    const int nbw = min((int)cross.x - 1, (int)mp.x + rad) - nx0 + 1;
-    const int nbh = min((int)cross.y - 1, (int)mp.y + rad) - ny0 + 1;
-    eval_box<T, NT>(ps, rs, s, cross, nx0, ny0, nbw, nbh, rrow, prow, boxv);
+    int nx0 = bx0, ny0 = by0, nbw = bw;
+    if (!merged) {
+        nx0 = max(0, (int)mp.x - rad);
+        ny0 = max(0, (int)mp.y - rad);
+        nbw = min((int)cross.x - 1, (int)mp.x + rad) - nx0 + 1;
+        const int nbh = min((int)cross.y - 1, (int)mp.y + rad) - ny0 + 1;
+        eval_box<T, NT>(ps, rs, s, cross, nx0, ny0, nbw, nbh, rrow, prow, boxv);
+    }
     for (int t = tid; t < k; t += NT) {
         const int i = t % fit, j = t / fit;
         const size2 c = clamp_pos((int)mp.x - rad + i, (int)mp.y - rad + j, cross);

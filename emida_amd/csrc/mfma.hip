@@ -9,8 +9,8 @@
 // reference rows y, where B_y is the Toeplitz matrix of reference row y (zero outside
 // the row).  All-zero Toeplitz tiles are skipped.
 //
-// Work decomposition: one workgroup = 32 patterns x 64 WX x-shifts (WX waves side by
-// side, two adjacent 32-wide tiles each) x 4 (4 / WX) consecutive y-shifts (4 per
+// Work decomposition: one workgroup = 32 patterns x 64 WX x-shifts (2 WX tiles of 32,
+// wave wx of a row taking tiles wx and wx + WX) x 4 (4 / WX) consecutive y-shifts (4 per
 // wave), WX = 4, 2 or 1 for pattern rows of up to 128, 64 or 32 columns, on a 1-D grid
 // whose order keeps the y-shift tiles of one (ROI, pattern tile) on one XCD.  It walks
 // the reference rows y; the pattern rows Y = y + sy of its y-shifts live in an LDS ring
@@ -148,16 +148,18 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
     const HT* refh = rhi + (size_t)roi * slice_elems;
     const HT* refl = SPLIT ? rlo + (size_t)roi * slice_elems : nullptr;
 
-    // this wave's two adjacent 32-wide x-shift tiles (the pair index rotates with the
-    // y-shift tile so that light edge and heavy centre pairs alternate per SIMD) and
-    // its 4 y-shifts sy0 + 4 wy + t
-    const int pair = (wave % WX + (int)syt) % WX, wy = wave / WX;
+    // this wave's two 32-wide x-shift tiles, wx and wx + WX, and its 4 y-shifts
+    // sy0 + 4 wy + t.  The non-zero Toeplitz tiles per x-shift tile fall off linearly
+    // from the centre (2, 4, .., 8, 8, .., 4, 2 K tiles for 128 columns), so tiles
+    // wx and wx + WX always add up to the same work and no wave waits for another
+    // at the per-row barriers.
+    const int wx = wave % WX, wy = wave / WX;
     const int syw = sy0 + TS * wy;
     int sx0[kMfmaTilesPerWave];
     bool active[kMfmaTilesPerWave];
 #pragma unroll
     for (int u = 0; u < kMfmaTilesPerWave; ++u) {
-        sx0[u] = -rx + (pair * kMfmaTilesPerWave + u) * 32;
+        sx0[u] = -rx + (wx + u * WX) * 32;
         active[u] = sx0[u] + rx < (int)cross.x;
     }
 

@@ -271,8 +271,8 @@ int configure(int argc, char** argv, run_config& c) {
         std::cerr << "Error: batchsize must be greater than 0.\n";
         return 1;
     }
-    if (a.count("loadworkers") && !parse_positive(a["loadworkers"], c.load_workers)) {
-        std::cerr << "Error: loadworkers must be greater than 0.\n";
+    if (a.count("loadworkers") && (!parse_positive(a["loadworkers"], c.load_workers) || c.load_workers > 64)) {
+        std::cerr << "Error: loadworkers must be in 1..64.\n";
         return 1;
     }
     c.analysis = a.count("analysis") != 0;

@@ -4,7 +4,8 @@
 // bytes are copied in file order into the raster, exactly what raw strip reads of an
 // uncompressed image produce (no byte swapping, as the reference).  Deliberate
 // differences: compressed images, other than 16-bit single-sample pixels and strip
-// data that would overflow the raster are rejected instead of copied as they are.
+// data that would overflow or not fill the raster are rejected instead of copied as
+// they are.
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -48,6 +49,7 @@ struct reader {
         const uint32_t type = u(e + 2, 2), count = u(e + 4, 4);
         const int sz = type == 3 ? 2 : (type == 4 ? 4 : 0);
         if (sz == 0) throw arg_error(name + ": unsupported TIFF field type " + std::to_string(type));
+        if (count == 0) throw arg_error(name + ": empty TIFF field");
         if (count > (1u << 26)) throw arg_error(name + ": TIFF field too long");
         std::vector<unsigned char> raw((size_t)count * sz);
         if (raw.size() <= 4)
@@ -130,6 +132,9 @@ void tiff_read(const char* path, uint16_t* raster, size2 size) {
             throw arg_error(name + ": truncated strip " + std::to_string(i));
         done += im.counts[i];
     }
+    // an uncompressed image fills the raster exactly; fewer bytes would leave the
+    // rest of the caller's buffer as it was
+    if (done != cap) throw arg_error(name + ": strip data are shorter than the image");
 }
 
 }  // namespace emida_amd

@@ -149,7 +149,7 @@ def _patch_short_tag(path, tag, value):
     raise AssertionError("tag not found")
 
 
-def test_orientation_and_oversized_strips_rejected(tmp_path):
+def test_orientation_and_bad_strip_tables_rejected(tmp_path):
     import struct
     r = np.full((8, 6), 7, np.uint16)
     p = str(tmp_path / "o.tif")
@@ -168,6 +168,26 @@ def test_orientation_and_oversized_strips_rejected(tmp_path):
             struct.pack_into("<I", b, e_ + 8, 6 * 8 * 2 + 2)
     open(q, "wb").write(bytes(b))
     assert _err(F.load_tiff, q, (6, 8)).code == C.ERR_ARG
+    # strip data shorter than the raster: rejected instead of leaving the buffer's tail
+    for i in range(struct.unpack_from("<H", b, ifd)[0]):
+        e_ = ifd + 2 + 12 * i
+        if struct.unpack_from("<H", b, e_)[0] == 279:
+            struct.pack_into("<I", b, e_ + 8, 6 * 8 * 2 - 2)
+    open(q, "wb").write(bytes(b))
+    e = _err(F.load_tiff, q, (6, 8))
+    assert e.code == C.ERR_ARG and "shorter than the image" in str(e)
+    # a field with no values (ImageWidth, count 0)
+    z = str(tmp_path / "zero.tif")
+    F.write_tiff(z, r)
+    b = bytearray(open(z, "rb").read())
+    ifd = struct.unpack_from("<I", b, 4)[0]
+    for i in range(struct.unpack_from("<H", b, ifd)[0]):
+        e_ = ifd + 2 + 12 * i
+        if struct.unpack_from("<H", b, e_)[0] == 256:
+            struct.pack_into("<I", b, e_ + 4, 0)
+    open(z, "wb").write(bytes(b))
+    e = _err(F.tiff_size, z)
+    assert e.code == C.ERR_ARG and "empty TIFF field" in str(e)
 
 
 # ------------------------------------------------------------------ slicepos / deformed list
@@ -261,7 +281,8 @@ def test_cli_help_and_missing_mandatory():
     (["-f", "4"], "the fitsize 4 is not supported"),
     (["--crosspolicy", "fast"], "Invalid crosspolicy argument."),
     (["--batchsize", "0"], "batchsize must be greater than 0"),
-    (["--loadworkers=-2"], "loadworkers must be greater than 0"),
+    (["--loadworkers=-2"], "loadworkers must be in 1..64"),
+    (["--loadworkers", "65"], "loadworkers must be in 1..64"),
     (["-c", "3"], "Argument parsing error."),
 ])
 def test_cli_argument_errors(tmp_path, extra, msg):

@@ -225,6 +225,30 @@ def test_cross_corr_fft_128_float_vs_oracle(s, ref_slices, batch):
     assert (np.abs(got - exp) / scale).max() < 2e-6 * np.log2(2 * s)
 
 
+@pytest.mark.parametrize("s", [64, 20, 33, 128, 96])
+@pytest.mark.parametrize("dt,tol", [(np.float64, 1e-12), (np.float32, 3e-6)])
+def test_cross_corr_fft_vs_rocfft(s, dt, tol):
+    # cross-check against rocFFT: torch.fft on a ROCm device runs hipFFT/rocFFT; the
+    # reference's R2C (2s x 2s, zero-padded), p * conj(r), unnormalised C2R
+    # (src/gpu_offset.hpp:303-316, src/cufft_helpers.hpp:105-155)
+    if dt == np.float64 and s > 64:
+        pytest.skip("double FFT slices are at most 64 wide")
+    import torch
+    rng = np.random.default_rng(31 + s)
+    ref_slices, batch = 3, 2
+    n = ref_slices * batch
+    pics = rng.standard_normal(s * s * n).astype(dt)
+    ref = rng.standard_normal(s * s * ref_slices).astype(dt)
+    got = K.run_cross_corr_fft(pics, ref, (s, s), ref_slices, batch).reshape(n, 2 * s, 2 * s).astype(np.float64)
+    dev = torch.device("cuda")
+    P = torch.fft.rfft2(torch.from_numpy(pics.reshape(n, s, s)).to(dev), s=(2 * s, 2 * s))
+    R = torch.fft.rfft2(torch.from_numpy(ref.reshape(ref_slices, s, s)).to(dev), s=(2 * s, 2 * s))
+    R = R.repeat(batch, 1, 1)  # slice i correlates with reference slice i % ref_slices
+    exp = torch.fft.irfft2(P * torch.conj(R), s=(2 * s, 2 * s), norm="forward").cpu().numpy().astype(np.float64)
+    scale = np.abs(exp).reshape(n, -1).max(axis=1)[:, None, None]
+    assert (np.abs(got - exp) / scale).max() < tol * np.log2(2 * s)
+
+
 def test_cross_corr_fft_smoke_4x4():  # test/cross_corr_fft_test.cpp:8-42 (same inputs, with an assertion)
     pic = np.array([[1, 2, 3, 1], [4, 5, 6, 1], [7, 8, 9, 1], [7, 8, 9, 1]], float)
     ref = np.array([[6, 3, 2, 1], [12, 23, 3, 1], [1, 8, 9, 1], [7, 8, 9, 1]], float)

@@ -330,7 +330,7 @@ int emida_tiff_size(const char* path, emida_size2* size);
  * uncompressed 16-bit single-sample TIFF copied in file order into raster
  * (size.x * size.y uint16, row-major, no byte swapping).  EMIDA_ERR_ARG for a size
  * other than `size`, an orientation other than top-left, or strips that are missing,
- * truncated or larger than the raster; EMIDA_ERR_UNSUPPORTED for compressed images
+ * truncated, larger than the raster or do not fill it; EMIDA_ERR_UNSUPPORTED for compressed images
  * and other pixel formats (the reference copies their raw bytes regardless). */
 int emida_tiff_load(const char* path, uint16_t* raster, emida_size2 size);
 

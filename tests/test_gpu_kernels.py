@@ -225,26 +225,59 @@ def test_cross_corr_fft_128_float_vs_oracle(s, ref_slices, batch):
     assert (np.abs(got - exp) / scale).max() < 2e-6 * np.log2(2 * s)
 
 
+def _rocfft_corr(pics, ref, s, ref_slices, batch):
+    """The reference's R2C (2s x 2s, zero-padded), p * conj(r) and unnormalised C2R
+    (src/gpu_offset.hpp:303-316, src/cufft_helpers.hpp:105-155) with rocFFT through
+    the system hipFFT library, in the HIP runtime the library under test uses."""
+    import ctypes
+    from emida_amd.gpu_offset import DeviceBuffer
+    fft = ctypes.CDLL("libhipfft.so")
+    dt = pics.dtype
+    double = dt == np.float64
+    cdt = np.complex128 if double else np.complex64
+    n, m = ref_slices * batch, 2 * s
+    nc = m * (m // 2 + 1)
+    P = np.zeros((n, m, m), dt)
+    P[:, :s, :s] = pics.reshape(n, s, s)
+    R = np.zeros((n, m, m), dt)
+    R[:, :s, :s] = np.tile(ref.reshape(ref_slices, s, s), (batch, 1, 1))  # slice i uses ref i % ref_slices
+
+    def plan(kind):
+        h = ctypes.c_void_p()
+        dims = (ctypes.c_int * 2)(m, m)
+        assert fft.hipfftPlanMany(ctypes.byref(h), 2, dims, None, 1, 0, None, 1, 0, kind, n) == 0
+        return h
+
+    r2c, c2r = plan(0x6A if double else 0x2A), plan(0x6C if double else 0x2C)  # D2Z/R2C, Z2D/C2R
+    fwd = fft.hipfftExecD2Z if double else fft.hipfftExecR2C
+    inv = fft.hipfftExecZ2D if double else fft.hipfftExecC2R
+    spectra = []
+    for a in (P, R):
+        d_in, d_out = DeviceBuffer.from_array(a), DeviceBuffer(n * nc * np.dtype(cdt).itemsize)
+        assert fwd(r2c, d_in.ptr, d_out.ptr) == 0
+        spectra.append(d_out.download(cdt))
+    H = (spectra[0] * np.conj(spectra[1])).astype(cdt)
+    d_h, d_c = DeviceBuffer.from_array(H), DeviceBuffer(n * m * m * dt.itemsize)
+    assert inv(c2r, d_h.ptr, d_c.ptr) == 0
+    out = d_c.download(dt).reshape(n, m, m)
+    fft.hipfftDestroy(r2c)
+    fft.hipfftDestroy(c2r)
+    return out
+
+
 @pytest.mark.parametrize("s", [64, 20, 33, 128, 96])
 @pytest.mark.parametrize("dt,tol", [(np.float64, 1e-12), (np.float32, 3e-6)])
 def test_cross_corr_fft_vs_rocfft(s, dt, tol):
-    # cross-check against rocFFT: torch.fft on a ROCm device runs hipFFT/rocFFT; the
-    # reference's R2C (2s x 2s, zero-padded), p * conj(r), unnormalised C2R
-    # (src/gpu_offset.hpp:303-316, src/cufft_helpers.hpp:105-155)
+    # cross-check of the FFT correlation against rocFFT (used in this test only)
     if dt == np.float64 and s > 64:
         pytest.skip("double FFT slices are at most 64 wide")
-    import torch
     rng = np.random.default_rng(31 + s)
     ref_slices, batch = 3, 2
     n = ref_slices * batch
     pics = rng.standard_normal(s * s * n).astype(dt)
     ref = rng.standard_normal(s * s * ref_slices).astype(dt)
     got = K.run_cross_corr_fft(pics, ref, (s, s), ref_slices, batch).reshape(n, 2 * s, 2 * s).astype(np.float64)
-    dev = torch.device("cuda")
-    P = torch.fft.rfft2(torch.from_numpy(pics.reshape(n, s, s)).to(dev), s=(2 * s, 2 * s))
-    R = torch.fft.rfft2(torch.from_numpy(ref.reshape(ref_slices, s, s)).to(dev), s=(2 * s, 2 * s))
-    R = R.repeat(batch, 1, 1)  # slice i correlates with reference slice i % ref_slices
-    exp = torch.fft.irfft2(P * torch.conj(R), s=(2 * s, 2 * s), norm="forward").cpu().numpy().astype(np.float64)
+    exp = _rocfft_corr(pics, ref, s, ref_slices, batch).astype(np.float64)
     scale = np.abs(exp).reshape(n, -1).max(axis=1)[:, None, None]
     assert (np.abs(got - exp) / scale).max() < tol * np.log2(2 * s)
 

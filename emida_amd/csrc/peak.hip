@@ -280,7 +280,20 @@ __device__ inline T exact_corr(const T* __restrict__ p, const T* __restrict__ r,
     for (int y = max(-sy, 0); y < y_end; ++y) {
         const T* pr = p + (size_t)(y + sy) * s.x + sx;
         const T* qr = r + (size_t)y * s.x;
-        for (int x = x0; x < x_end; ++x) sum = fma(pr[x], qr[x], sum);
+        int x = x0;
+        // the operands of 8 products are loaded before their 8 dependent fmas, so one
+        // memory latency is paid per 8 terms; the summation order is unchanged
+        for (; x + 8 <= x_end; x += 8) {
+            T u[8], v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                u[i] = pr[x + i];
+                v[i] = qr[x + i];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) sum = fma(u[i], v[i], sum);
+        }
+        for (; x < x_end; ++x) sum = fma(pr[x], qr[x], sum);
     }
     return sum;
 }

@@ -180,6 +180,45 @@ def test_refined_equals_brute_policy_edge_peaks(cross):
         np.testing.assert_array_equal(g, e)
 
 
+def _spread_patterns(batch):
+    """384 x 128 uint16 patterns of three 128 x 128 ROIs: vertical stripes (a ridge of
+    candidates along the y-shifts), independent noise in pattern and reference (a weak
+    peak: candidates scattered over a box of about 30 x 30 to 70 x 70 shifts depending
+    on the mode's rounding bound, more than one box evaluation holds) and diagonal
+    stripes with noise (a compact peak)."""
+    rng = np.random.default_rng(5)
+    yy, xx = np.mgrid[0:128, 0:384].astype(float)
+
+    def one():
+        a = 30000 + 20000 * np.sin(2 * np.pi * xx / 16)
+        a[:, 128:256] = 30000 + rng.normal(0, 3000, (128, 128))
+        a[:, 256:] = (30000 + 15000 * np.sin(2 * np.pi * (xx[:, 256:] + yy[:, 256:]) / 24)
+                      + rng.normal(0, 2000, (128, 128)))
+        return np.clip(a, 0, 65535).astype(np.uint16)
+
+    ref = one()
+    pics = np.stack([one() for _ in range(batch)])
+
+    class P:
+        pass
+
+    ps = P()
+    ps.ref, ps.pics, ps.size = ref, pics, (384, 128)
+    return ps, [(0, 0), (128, 0), (256, 0)]
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+@pytest.mark.parametrize("mode", list(MODES))
+def test_refined_equals_brute_policy_spread_candidates(T, mode):
+    # candidate sets wider than one box are evaluated one chain per candidate, from the
+    # prepared slices in HBM; results still the brute policy's
+    ps, begins = _spread_patterns(2)
+    got = _run(ps, begins, (128, 128), (255, 255), 7, 2, T=T, mode=mode)
+    exp = _run(ps, begins, (128, 128), (255, 255), 7, 2, T=T, policy=emida_amd.CROSS_POLICY_BRUTE)
+    for g, e, what in zip(got, exp, ("offsets", "coefs", "maxes", "neighbourhoods")):
+        np.testing.assert_array_equal(g, e, err_msg=what)
+
+
 @pytest.mark.slow
 def test_precision_form_workload_subset_vs_fp64_oracle():
     # the precision-form workload's shape: 873 x 873 patterns, ROIs of 128 x 128 on a

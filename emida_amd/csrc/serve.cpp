@@ -9,7 +9,8 @@
 // are queued (up to max_batch) as soon as the plan has a free slot, i.e. a device
 // buffer set for one batch (patterns, working memory, results); new slots are created
 // while the sum of slot sizes stays within the HBM budget, so admission is by
-// memory.  Batches go round-robin over several streams and overlap on the device.
+// memory; when other plans' idle slots fill the budget, the one idle longest is freed
+// for a plan that has requests waiting.  Batches go round-robin over several streams and overlap on the device.
 // Each (slot, batch size) pipeline - kernels plus the copies of the results into the
 // slot's pinned buffers - is captured once in a hipGraph and replayed; batch sizes
 // are rounded up to a power of two for this, the extra slices computing on stale
@@ -143,13 +144,7 @@ struct emida_server_s {
             if (t.joinable()) t.join();
         (void)hipSetDevice(device);
         for (auto& s : streams) (void)hipStreamSynchronize(s);
-        for (auto& sl : slots) {
-            for (auto& g : sl->graphs) (void)hipGraphExecDestroy(g.second);
-            if (sl->block) (void)hipFree(sl->block);
-            if (sl->h_offsets) (void)hipHostFree(sl->h_offsets);
-            if (sl->h_coefs) (void)hipHostFree(sl->h_coefs);
-            if (sl->done) (void)hipEventDestroy(sl->done);
-        }
+        for (auto& sl : slots) release_slot(*sl);
         for (void* p : staging_all) (void)hipHostFree(p);
         for (auto& kv : requests) delete kv.second;
         for (auto& s : streams) (void)hipStreamDestroy(s);
@@ -185,6 +180,18 @@ struct emida_server_s {
         plans.push_back(std::move(p));
         st.device_bytes_resident += plans.back()->off->device_bytes;
         return (uint32_t)plans.size() - 1;
+    }
+
+    static void release_slot(slot& sl) {
+        for (auto& g : sl.graphs) (void)hipGraphExecDestroy(g.second);
+        sl.graphs.clear();
+        if (sl.block) (void)hipFree(sl.block);
+        if (sl.h_offsets) (void)hipHostFree(sl.h_offsets);
+        if (sl.h_coefs) (void)hipHostFree(sl.h_coefs);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        sl.block = nullptr;
+        sl.h_offsets = sl.h_coefs = nullptr;
+        sl.done = nullptr;
     }
 
     slot* new_slot(uint32_t pi) {
@@ -298,12 +305,74 @@ struct emida_server_s {
         return best;
     }
 
+    // A plan with queued requests that cannot get a slot although it is below its own
+    // slot limit: the budget is taken (by other plans' slots).  Oldest head first.
+    int starving_plan() const {
+        int best = -1;
+        uint64_t t = ~0ull;
+        for (size_t i = 0; i < plans.size(); ++i) {
+            const plan_rt& p = *plans[i];
+            if (p.queue.empty() || !p.free_slots.empty()) continue;
+            if (cfg.slots_per_plan != 0 && p.n_slots >= cfg.slots_per_plan) continue;
+            if (slot_bytes_total + p.slot_bytes <= cfg.hbm_budget) continue;
+            if (p.queue.front()->submit_ns < t) {
+                t = p.queue.front()->submit_ns;
+                best = (int)i;
+            }
+        }
+        return best;
+    }
+
+    // Another plan holding an idle slot, preferably one with nothing queued; -1 when the
+    // budget is held by slots in flight only (their completions free them).
+    int victim_plan(int starving) const {
+        int v = -1;
+        for (size_t i = 0; i < plans.size(); ++i) {
+            if ((int)i == starving || plans[i]->free_slots.empty()) continue;
+            if (v < 0 || (plans[i]->queue.empty() && !plans[v]->queue.empty())) v = (int)i;
+        }
+        return v;
+    }
+
+    // Frees the idle slot of plan v that has been idle longest, giving its bytes back to
+    // the budget: without this, plans whose idle slots fill the budget would keep the
+    // requests of every other plan waiting forever.
+    void evict_slot(int v, std::unique_lock<std::mutex>& lock) {
+        plan_rt& p = *plans[v];
+        slot* sl = p.free_slots.front();
+        p.free_slots.erase(p.free_slots.begin());
+        --p.n_slots;
+        slot_bytes_total -= sl->bytes;
+        std::unique_ptr<slot> owned;
+        for (auto it = slots.begin(); it != slots.end(); ++it)
+            if (it->get() == sl) {
+                owned = std::move(*it);
+                slots.erase(it);
+                break;
+            }
+        st.slots = (uint32_t)slots.size();
+        st.device_bytes_batches = slot_bytes_total;
+        lock.unlock();
+        release_slot(*owned);  // idle: no work of it is in flight
+        lock.lock();
+    }
+
     void schedule_loop() {
         std::unique_lock<std::mutex> lock(mu);
         while (true) {
-            int pi = -1;
-            cv_sched.wait(lock, [&] { return stop || (pi = pick_plan()) >= 0; });
+            int pi = -1, victim = -1;
+            cv_sched.wait(lock, [&] {
+                if (stop) return true;
+                if ((pi = pick_plan()) >= 0) return true;
+                const int sp = starving_plan();
+                victim = sp >= 0 ? victim_plan(sp) : -1;
+                return victim >= 0;
+            });
             if (stop) return;
+            if (pi < 0) {
+                evict_slot(victim, lock);
+                continue;
+            }
             plan_rt& p = *plans[pi];
             slot* sl;
             if (!p.free_slots.empty()) {
@@ -403,10 +472,10 @@ struct emida_server_s {
             if (inflight[k].empty()) return;  // stop with nothing left on this stream
             slot* sl = inflight[k].front().sl;
             std::vector<request*> reqs = inflight[k].front().reqs;
+            plan_rt& p = *plans[sl->plan];  // plan objects stay put; the vector may grow
             lock.unlock();
             EMIDA_HIP(hipEventSynchronize(sl->done));
             const uint64_t t = now_ns();
-            const plan_rt& p = *plans[sl->plan];
             const size_t nb = p.off->nb;
             for (size_t i = 0; i < reqs.size(); ++i) {
                 if (reqs[i]->out_offsets)
@@ -425,7 +494,7 @@ struct emida_server_s {
             }
             st.completed += reqs.size();
             inflight_bytes -= sl->bytes;
-            plans[sl->plan]->free_slots.push_back(sl);
+            p.free_slots.push_back(sl);
             cv_sched.notify_one();
             cv_done.notify_all();
             cv_staging.notify_all();

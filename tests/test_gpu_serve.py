@@ -3,7 +3,8 @@
 Each request's offsets must be exactly those of the plain pipeline on the same
 pattern (GpuOffset with batch 1), whether the request ran alone or mixed with
 requests of other plans submitted concurrently from several threads, with and
-without hipGraph replays, and under a small HBM budget.
+without hipGraph replays, under a small HBM budget, and when one plan's idle slots
+must be freed for another's.
 """
 import threading
 
@@ -142,6 +143,35 @@ def test_small_budget_still_completes(workload):
         st = srv.stats()
         assert 0 < st["device_bytes_batches"] <= budget
         assert st["device_bytes_peak_inflight"] <= st["device_bytes_batches"]
+
+
+def test_idle_slots_are_given_back_to_a_waiting_plan(workload):
+    # slots_per_plan 0 lets one plan take the whole budget; a budget that holds one slot
+    # of either plan but never both makes every switch of plan free the other plan's
+    # idle slot (before, the second plan's request waited forever)
+    ps, plans = workload
+    two = plans[:2]
+    sizes = []
+    for p in two:
+        with emida_amd.Server(hbm_budget=1 << 30, max_batch=1, streams=1, slots_per_plan=0) as probe:
+            pid = probe.add_plan(p["T"], ps.ref, p["begins"], (p["side"], p["side"]), p["cross"], p["pol"], p["fit"])
+            probe.wait(probe.submit(pid, ps.pics[0])[0])
+            sizes.append(probe.stats()["device_bytes_batches"])
+    budget = max(sizes)
+    assert sum(sizes) > budget
+    srv, ids = _server(ps, two, hbm_budget=budget, max_batch=1, streams=2, slots_per_plan=0)
+    with srv:
+        results = {}
+        for _ in range(2):
+            for pi in range(2):
+                for i in range(3):
+                    rid, offs, coefs = srv.submit(ids[pi], ps.pics[i])
+                    srv.wait(rid)
+                    results[(pi, i)] = (offs, coefs)
+        _check(two, results)
+        st = srv.stats()
+        assert st["completed"] == st["submitted"] == 12
+        assert st["slots"] == 1 and 0 < st["device_bytes_batches"] <= budget
 
 
 def test_dropped_output_arrays_stay_alive(workload):

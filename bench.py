@@ -556,7 +556,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(patterns shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(patterns would shard across GPUs with no exchange between them)", file=sys.stderr)
         return 2
     if args.copy_calibration_mb:
         copy_calibration(args.copy_calibration_mb)

@@ -1,7 +1,7 @@
 """numpy front end of the CPU oracle (oracle/oracle.cpp).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
-checker and by bench.py's cpu_baseline leg. The product library never uses it.
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to verify
+its result and by bench.py's cpu_baseline leg. The product library never uses it.
 
 What pins each oracle function (besides the reference file:line it restates):
 

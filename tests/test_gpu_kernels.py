@@ -146,15 +146,15 @@ def test_cross_corr_64_fixture():  # test/cross_corr_test.cpp:205-217
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 @pytest.mark.parametrize("size,res,ref_slices,batch", [
     ((32, 32), (63, 63), 3, 2),
-    ((17, 23), (33, 45), 2, 1),
-    ((64, 64), (9, 9), 2, 2),
-    ((128, 128), (255, 255), 1, 1),
+    ((17, 23), (33, 45), 2, 1),      # the x4 kernel's last thread of a row owns one shift
+    ((64, 64), (9, 9), 2, 2),        # double: thread-per-shift kernel over LDS (x4 rows do not fit)
+    ((128, 128), (255, 255), 1, 1),  # thread-per-shift kernel from HBM (slices larger than LDS)
     ((64, 48), (127, 95), 2, 17),    # two pattern tiles of the register-blocked kernel, the second partial
     ((40, 24), (81, 49), 1, 3),      # the widest window the reference allows (2 size + 1)
     ((24, 16), (32, 20), 2, 5),      # even window sizes
     ((100, 30), (199, 59), 1, 2),    # 3 waves along x-shifts
-    ((136, 8), (15, 15), 1, 2),      # wider than 128: thread-per-shift kernel
-    ((128, 4), (257, 7), 1, 1),      # window wider than 256: thread-per-shift kernel
+    ((136, 8), (15, 15), 1, 2),      # wider than 128: not the register-blocked kernel
+    ((128, 4), (257, 7), 1, 1),      # window wider than 256: not the register-blocked kernel
 ])
 def test_cross_corr_bitexact_vs_oracle(dt, size, res, ref_slices, batch):
     rng = np.random.default_rng(13)

@@ -8,8 +8,8 @@
 //
 // Three kernels: a register-blocked one (below, "tiled") for slices up to 128 pixels
 // wide, correlation windows up to 256 wide and launches of at least 256 workgroups;
-// for other shapes whose slices fit in LDS, one with four x-shifts per thread sliding
-// a register window over zero-padded rows ("x4"); and a thread-per-shift one for the
+// for other shapes whose slices fit in LDS, one with eight x-shifts per thread sliding
+// a register window over zero-padded rows ("x8"); and a thread-per-shift one for the
 // rest: one workgroup per (slice, 4-row band of shifts, 64-column band of shifts),
 // staged in LDS when both slices fit in 64 KiB.
 #include "kernels.hpp"
@@ -65,71 +65,72 @@ __global__ __launch_bounds__(kBruteTX* kBruteTY) void cross_corr_kernel(const T*
     res[(size_t)sn * res_size.area() + (size_t)ry * res_size.x + rx] = sum;
 }
 
-// The same sums, four consecutive x-shifts per thread, for slices staged in LDS: the
-// pattern rows are stored with 3 zeros on each side, so a 4-value window slides along
-// the row (one new pattern value and one reference value per step feed 4 fmas).  Each
-// thread walks the union of its 4 shifts' overlaps; a term outside a shift's own
+// The same sums, NX consecutive x-shifts per thread, for slices staged in LDS: the
+// pattern rows are stored with NX - 1 zeros on each side, so an NX-value window slides
+// along the row (one new pattern value and one reference value per step feed NX fmas).
+// Each thread walks the union of its shifts' overlaps; a term outside a shift's own
 // overlap multiplies an exact zero and leaves its sum unchanged (finite values), and
 // every sum still sees its own terms y outer, x inner.  Indices: the window reads
-// pattern column x + sx0 + t, t < 4, for x in the union, which stays in [-3, S + 3).
-constexpr int kX4 = 4;             // x-shifts per thread
-constexpr int kX4Pad = kX4 - 1;    // zeros on each side of a staged pattern row
-constexpr int kX4TX = 16, kX4TY = 16;
+// pattern column x + sx0 + t, t < NX, for x in the union, which stays in
+// [-(NX - 1), S + NX - 1).
+constexpr int kXN = 8;  // x-shifts per thread (4: 0.146 ms, 16: 0.167 ms at 32 x 32 slices, 63 x 63 windows; 8: 0.130 ms)
+constexpr int kXNTX = 64 / kXN, kXNTY = 256 / kXNTX;
 
 template <class T>
-__host__ __device__ inline size_t x4_lds_elems(size2 size) {
-    return (size_t)size.y * (size.x + 2 * kX4Pad) + (size_t)size.y * size.x;
+__host__ __device__ inline size_t xn_lds_elems(size2 size) {
+    return (size_t)size.y * (size.x + 2 * (kXN - 1)) + (size_t)size.y * size.x;
 }
 
 template <class T>
-__global__ __launch_bounds__(kX4TX* kX4TY) void cross_corr_x4_kernel(const T* __restrict__ pics,
+__global__ __launch_bounds__(kXNTX* kXNTY) void cross_corr_x8_kernel(const T* __restrict__ pics,
                                                                     const T* __restrict__ ref,
                                                                     T* __restrict__ res, size2 size,
                                                                     size2 res_size, uint32_t ref_slices) {
+    constexpr int NX = kXN, PAD = kXN - 1;
     extern __shared__ unsigned char smem_raw[];
     const uint32_t sn = blockIdx.z;
-    const int S = (int)size.x, SY = (int)size.y, W = S + 2 * kX4Pad;
+    const int S = (int)size.x, SY = (int)size.y, W = S + 2 * PAD;
     T* sp = reinterpret_cast<T*>(smem_raw);  // SY rows of W: zeros, pattern row, zeros
     T* sq = sp + (size_t)SY * W;             // SY rows of S
     const T* p = pics + (size_t)sn * size.area();
     const T* q = ref + (size_t)(sn % ref_slices) * size.area();
-    const int tid = threadIdx.y * kX4TX + threadIdx.x;
-    for (int i = tid; i < SY * W; i += kX4TX * kX4TY) {
-        const int y = i / W, c = i % W - kX4Pad;
+    const int tid = threadIdx.y * kXNTX + threadIdx.x;
+    for (int i = tid; i < SY * W; i += kXNTX * kXNTY) {
+        const int y = i / W, c = i % W - PAD;
         sp[i] = (c >= 0 && c < S) ? p[(size_t)y * S + c] : T(0);
     }
-    for (int i = tid; i < SY * S; i += kX4TX * kX4TY) sq[i] = q[i];
+    for (int i = tid; i < SY * S; i += kXNTX * kXNTY) sq[i] = q[i];
     __syncthreads();
-    const uint32_t rx0 = (blockIdx.x * kX4TX + threadIdx.x) * kX4;
-    const uint32_t ry = blockIdx.y * kX4TY + threadIdx.y;
+    const uint32_t rx0 = (blockIdx.x * kXNTX + threadIdx.x) * NX;
+    const uint32_t ry = blockIdx.y * kXNTY + threadIdx.y;
     if (rx0 >= res_size.x || ry >= res_size.y) return;
-    const int nsh = min(kX4, (int)(res_size.x - rx0));  // shifts of this thread in the window
+    const int nsh = min(NX, (int)(res_size.x - rx0));  // shifts of this thread in the window
     const int sx0 = (int)rx0 - (int)((res_size.x - 1) / 2);
     const int sy = (int)ry - (int)((res_size.y - 1) / 2);
     const int xu0 = max(0, -(sx0 + nsh - 1)), xu1 = min(S, S - sx0);  // union of the overlaps
-    T acc[kX4];
+    T acc[NX];
 #pragma unroll
-    for (int t = 0; t < kX4; ++t) acc[t] = T(0);
+    for (int t = 0; t < NX; ++t) acc[t] = T(0);
     if (xu0 < xu1) {
         for (int y = max(-sy, 0); y < min(SY, SY - sy); ++y) {
-            const T* pr = sp + (size_t)(y + sy) * W + kX4Pad + sx0;  // pr[x + t]: column x + sx0 + t
+            const T* pr = sp + (size_t)(y + sy) * W + PAD + sx0;  // pr[x + t]: column x + sx0 + t
             const T* qr = sq + (size_t)y * S;
-            T w0 = pr[xu0], w1 = pr[xu0 + 1], w2 = pr[xu0 + 2];
+            T w[NX];
+#pragma unroll
+            for (int t = 0; t < NX - 1; ++t) w[t] = pr[xu0 + t];
             for (int x = xu0; x < xu1; ++x) {
-                const T w3 = pr[x + 3], qv = qr[x];
-                acc[0] = mac<T>(w0, qv, acc[0]);
-                acc[1] = mac<T>(w1, qv, acc[1]);
-                acc[2] = mac<T>(w2, qv, acc[2]);
-                acc[3] = mac<T>(w3, qv, acc[3]);
-                w0 = w1;
-                w1 = w2;
-                w2 = w3;
+                w[NX - 1] = pr[x + NX - 1];
+                const T qv = qr[x];
+#pragma unroll
+                for (int t = 0; t < NX; ++t) acc[t] = mac<T>(w[t], qv, acc[t]);
+#pragma unroll
+                for (int t = 0; t < NX - 1; ++t) w[t] = w[t + 1];
             }
         }
     }
     T* o = res + (size_t)sn * res_size.area() + (size_t)ry * res_size.x + rx0;
 #pragma unroll
-    for (int t = 0; t < kX4; ++t)
+    for (int t = 0; t < NX; ++t)
         if (t < nsh) o[t] = acc[t];
 }
 
@@ -319,8 +320,8 @@ const char* cross_corr_kernel_name(bool is_double, size2 size, size2 res_size, u
     const uint64_t n = is_double ? tiled_workgroups<double>(size, res_size, ref_slices, batch, sh)
                                  : tiled_workgroups<float>(size, res_size, ref_slices, batch, sh);
     if (n) return "cross_corr_tiled_kernel";
-    const size_t x4 = is_double ? x4_lds_elems<double>(size) * sizeof(double) : x4_lds_elems<float>(size) * sizeof(float);
-    return x4 <= kBruteLds ? "cross_corr_x4_kernel" : "cross_corr_kernel";
+    const size_t xn = is_double ? xn_lds_elems<double>(size) * sizeof(double) : xn_lds_elems<float>(size) * sizeof(float);
+    return xn <= kBruteLds ? "cross_corr_x8_kernel" : "cross_corr_kernel";
 }
 
 template <class T>
@@ -342,12 +343,12 @@ void launch_cross_corr(const T* pics, const T* ref, T* res, size2 size, size2 re
         return;
     }
     if (total > 65535u) throw arg_error("cross_corr: more than 65535 slices in one launch");
-    if (x4_lds_elems<T>(size) * sizeof(T) <= kBruteLds) {
-        dim3 block(kX4TX, kX4TY);
-        dim3 grid(div_up(res_size.x, (uint32_t)(kX4TX * kX4)), div_up(res_size.y, (uint32_t)kX4TY), total);
-        cross_corr_x4_kernel<T><<<grid, block, x4_lds_elems<T>(size) * sizeof(T), st>>>(pics, ref, res, size,
+    if (xn_lds_elems<T>(size) * sizeof(T) <= kBruteLds) {
+        dim3 block(kXNTX, kXNTY);
+        dim3 grid(div_up(res_size.x, (uint32_t)(kXNTX * kXN)), div_up(res_size.y, (uint32_t)kXNTY), total);
+        cross_corr_x8_kernel<T><<<grid, block, xn_lds_elems<T>(size) * sizeof(T), st>>>(pics, ref, res, size,
                                                                                         res_size, ref_slices);
-        EMIDA_LAUNCHED("cross_corr_x4_kernel");
+        EMIDA_LAUNCHED("cross_corr_x8_kernel");
         return;
     }
     dim3 block(kBruteTX, kBruteTY);

@@ -146,8 +146,8 @@ def test_cross_corr_64_fixture():  # test/cross_corr_test.cpp:205-217
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 @pytest.mark.parametrize("size,res,ref_slices,batch", [
     ((32, 32), (63, 63), 3, 2),
-    ((17, 23), (33, 45), 2, 1),      # the x4 kernel's last thread of a row owns one shift
-    ((64, 64), (9, 9), 2, 2),        # double: thread-per-shift kernel over LDS (x4 rows do not fit)
+    ((17, 23), (33, 45), 2, 1),      # the x8 kernel's last thread of a row owns one shift
+    ((64, 64), (9, 9), 2, 2),        # double: thread-per-shift kernel over LDS (x8 rows do not fit)
     ((128, 128), (255, 255), 1, 1),  # thread-per-shift kernel from HBM (slices larger than LDS)
     ((64, 48), (127, 95), 2, 17),    # two pattern tiles of the register-blocked kernel, the second partial
     ((40, 24), (81, 49), 1, 3),      # the widest window the reference allows (2 size + 1)

@@ -47,11 +47,13 @@ void launch_subpixel(const T* neigh, int s, uint32_t n, const data_index<T>* max
 // Precision form: exact peak from a float matrix-core screen of `total` slices and the
 // T-precision prepared pattern (p, [slice][s]) and reference (r, [roi][s]) slices;
 // outputs as launch_peak with the brute position policy.  The candidate bound is
-// tau = tol_rel ||p|| ||r|| + tol_abs (||p|| + ||r||) per slice.
+// tau = tol_rel ||p|| ||r|| + tol_abs (||p|| + ||r||) per slice.  screen_max (optional):
+// the screen maximum of each slice as an ordered code (screen_max_code), from
+// launch_mfma_corr; without it the kernel scans the screen for its maximum.
 template <class T>
 void launch_refine(const float* screen, size_t corr_len, const T* p, const T* r, size2 s, size2 cross, uint32_t nb,
                    double tol_rel, double tol_abs, int fit, data_index<T>* maxes, T* neigh, double* offsets,
-                   double* coefs, uint32_t total, hipStream_t st);
+                   double* coefs, uint32_t total, hipStream_t st, const uint32_t* screen_max = nullptr);
 // Screen error bound of launch_mfma_corr against the T-precision brute values, for
 // launch_refine (both values' rounding, counted twice): mode as emida_mfma_mode.
 void mfma_refine_bounds(int mode, size2 s, bool t_double, double& tol_rel, double& tol_abs);
@@ -90,14 +92,33 @@ void launch_hadamard(cplx<T>* pics, const cplx<T>* ref, size2 one, uint32_t ref_
 
 // ---- mfma.hip (matrix-core brute correlation, float results)
 // Prepared slices as half-type hi (+ lo when lo != nullptr) planes [slice][y][x];
-// corr: [pattern * nb + roi][cross.y][cross.x].
+// corr: [pattern * nb + roi][cross.y][cross.x].  screen_max (optional, [pattern * nb +
+// roi], zero on entry): receives the largest value of each slice (NaN ignored) as an
+// ordered code, atomically.
 template <class HT>
 void launch_mfma_corr(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
-                      uint32_t nb, uint32_t npat, hipStream_t st);
+                      uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* screen_max = nullptr);
 void check_mfma_shape(size2 s, size2 cross);
+// zero (optional): one uint32 per slice set to 0 (the screen maxima of the next
+// launch_mfma_corr).
 template <class IN, class S, class HT>
 void launch_prepare_split(const IN* pics, HT* hi, HT* lo, const float* hx, const float* hy, const S* sums,
-                          const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st);
+                          const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st,
+                          uint32_t* zero = nullptr);
+
+// Order-preserving uint32 code of a float (a < b  <=>  code(a) < code(b), -0 < +0);
+// 0 is below every code and stands for "no value".
+__host__ __device__ inline uint32_t screen_max_code(float v) {
+    uint32_t u;
+    __builtin_memcpy(&u, &v, 4);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ inline float screen_max_value(uint32_t c) {
+    const uint32_t u = (c & 0x80000000u) ? (c & 0x7fffffffu) : ~c;
+    float v;
+    __builtin_memcpy(&v, &u, 4);
+    return v;
+}
 template <class HT>
 void launch_split(const float* v, HT* hi, HT* lo, size_t n, hipStream_t st);
 

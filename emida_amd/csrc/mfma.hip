@@ -65,13 +65,14 @@ __global__ __launch_bounds__(256) void prepare_split_kernel(const IN* __restrict
                                                             HT* __restrict__ lo, const float* __restrict__ hx,
                                                             const float* __restrict__ hy, const S* __restrict__ sums,
                                                             const size2* __restrict__ begins, size2 src, size2 slice,
-                                                            uint32_t nb, uint32_t total) {
+                                                            uint32_t nb, uint32_t total, uint32_t* __restrict__ zero) {
 #pragma clang fp contract(off)
     const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t area = slice.area();
     const size_t sn = tid / area;
     if (sn >= total) return;
     const uint32_t st = (uint32_t)(tid % area);
+    if (zero && st == 0) zero[sn] = 0u;
     const uint32_t x = st % slice.x, y = st / slice.x;
     const size2 b0 = begins[sn % nb];
     float v = (float)pics[(sn / nb) * src.area() + (size_t)(b0.y + y) * src.x + b0.x + x];
@@ -122,7 +123,8 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
                                                                 const HT* __restrict__ rhi,
                                                                 const HT* __restrict__ rlo,
                                                                 float* __restrict__ corr, size2 s, size2 cross,
-                                                                uint32_t nb, uint32_t npat, uint32_t n_wg) {
+                                                                uint32_t nb, uint32_t npat, uint32_t n_wg,
+                                                                uint32_t* __restrict__ smax) {
     using P = mfma_plan<HT, SPLIT, KMAX, WX>;
     using V = typename mfma_vec<HT>::type;
     constexpr int NT = kMfmaNT, TS = kMfmaTS, ROWS = P::ROWS;
@@ -290,6 +292,43 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
         }
         __syncthreads();
     }
+    // per-slice maximum of the values written below (NaN never wins a strict compare):
+    // per accumulator row j the lane's largest value over its y-shifts and tiles, then
+    // over the 32 lanes of the half (xor < 32 stays in the half); lane j < 16 of each
+    // half publishes pattern row 8 (j/4) + 4 (l/32) + j%4 with one atomic max
+    if (smax) {
+        float mx[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) mx[j] = -__builtin_huge_valf();
+#pragma unroll
+        for (int t = 0; t < TS; ++t) {
+            const bool yin = syw + t + ry < (int)cross.y;
+#pragma unroll
+            for (int u = 0; u < kMfmaTilesPerWave; ++u) {
+                const bool in = yin && sx0[u] + m_a + rx < (int)cross.x;
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const float v = acc[t][u][j] * mfma_vec<HT>::out;
+                    if (in && v > mx[j]) mx[j] = v;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+#pragma unroll
+            for (int off = 16; off > 0; off >>= 1) {
+                const float o = __shfl_xor(mx[j], off);
+                if (o > mx[j]) mx[j] = o;
+            }
+        float mine = -__builtin_huge_valf();
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            if (j == m_a) mine = mx[j];
+        if (m_a < 16 && mine > -__builtin_huge_valf()) {
+            const uint32_t p = p0 + 8 * (m_a / 4) + 4 * half + (m_a % 4);
+            if (p < npat) atomicMax(&smax[(size_t)p * nb + roi], screen_max_code(mine));
+        }
+    }
     // write the accumulators: D[8 (j/4) + 4 (l/32) + j%4][l%32]
 #pragma unroll
     for (int t = 0; t < TS; ++t) {
@@ -314,7 +353,7 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
 // --------------------------------------------------------------- launchers
 template <class HT, bool SPLIT, int KMAX, int WX>
 void launch_mfma_corr_g(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
-                        uint32_t nb, uint32_t npat, hipStream_t st) {
+                        uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* smax) {
     using P = mfma_plan<HT, SPLIT, KMAX, WX>;
     static_assert(P::lds_bytes <= 80 * 1024, "two workgroups per CU");
     auto kern = mfma_corr_kernel<HT, SPLIT, KMAX, WX>;
@@ -325,20 +364,20 @@ void launch_mfma_corr_g(const HT* phi, const HT* plo, const HT* rhi, const HT* r
     const uint64_t n_wg = (uint64_t)nb * div_up(npat, (uint32_t)kMfmaM) * div_up(cross.y, (uint32_t)P::ROWS);
     if (n_wg > (1u << 24)) throw unsupported_error("matrix-core brute policy: too many workgroups in one launch");
     kern<<<(uint32_t)n_wg, kMfmaNT, P::lds_bytes, st>>>(phi, plo, rhi, rlo, corr, s, cross, nb, npat,
-                                                        (uint32_t)n_wg);
+                                                        (uint32_t)n_wg, smax);
     EMIDA_LAUNCHED("mfma_corr_kernel");
 }
 
 // Row width class of a slice: 32, 64 or 128 columns (windows up to 64, 128, 256 wide).
 template <class HT, bool SPLIT>
 void launch_mfma_corr_t(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
-                        uint32_t nb, uint32_t npat, hipStream_t st) {
+                        uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* smax) {
     if (s.x <= 32)
-        launch_mfma_corr_g<HT, SPLIT, 32, 1>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st);
+        launch_mfma_corr_g<HT, SPLIT, 32, 1>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax);
     else if (s.x <= 64)
-        launch_mfma_corr_g<HT, SPLIT, 64, 2>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st);
+        launch_mfma_corr_g<HT, SPLIT, 64, 2>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax);
     else
-        launch_mfma_corr_g<HT, SPLIT, 128, 4>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st);
+        launch_mfma_corr_g<HT, SPLIT, 128, 4>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax);
 }
 
 void check_mfma_shape(size2 s, size2 cross) {
@@ -350,13 +389,13 @@ void check_mfma_shape(size2 s, size2 cross) {
 
 template <class HT>
 void launch_mfma_corr(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
-                      uint32_t nb, uint32_t npat, hipStream_t st) {
+                      uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* smax) {
     check_mfma_shape(s, cross);
     if (nb == 0 || npat == 0) return;
     if (plo)
-        launch_mfma_corr_t<HT, true>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st);
+        launch_mfma_corr_t<HT, true>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax);
     else
-        launch_mfma_corr_t<HT, false>(phi, nullptr, rhi, nullptr, corr, s, cross, nb, npat, st);
+        launch_mfma_corr_t<HT, false>(phi, nullptr, rhi, nullptr, corr, s, cross, nb, npat, st, smax);
 }
 
 // |screen - T-precision brute value| <= tol_rel ||p|| ||r|| + tol_abs (||p|| + ||r||)
@@ -387,11 +426,13 @@ void mfma_refine_bounds(int mode, size2 s, bool t_double, double& tol_rel, doubl
 
 template <class IN, class S, class HT>
 void launch_prepare_split(const IN* pics, HT* hi, HT* lo, const float* hx, const float* hy, const S* sums,
-                          const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st) {
+                          const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st,
+                          uint32_t* zero) {
     const size_t n = (size_t)slice.area() * total;
     if (n == 0) return;
     prepare_split_kernel<IN, S, HT><<<(unsigned)div_up(n, (size_t)256), 256, 0, st>>>(pics, hi, lo, hx, hy, sums,
-                                                                                     begins, src, slice, nb, total);
+                                                                                     begins, src, slice, nb, total,
+                                                                                     zero);
     EMIDA_LAUNCHED("prepare_split_kernel");
 }
 
@@ -415,17 +456,17 @@ void launch_split(const float* v, HT* hi, HT* lo, size_t n, hipStream_t st) {
 
 #define INST_MFMA(HT)                                                                                          \
     template void launch_mfma_corr<HT>(const HT*, const HT*, const HT*, const HT*, float*, size2, size2, uint32_t, \
-                                       uint32_t, hipStream_t);                                                 \
+                                       uint32_t, hipStream_t, uint32_t*);                                      \
     template void launch_split<HT>(const float*, HT*, HT*, size_t, hipStream_t);                               \
     template void launch_prepare_split<uint16_t, uint32_t, HT>(const uint16_t*, HT*, HT*, const float*,        \
                                                                const float*, const uint32_t*, const size2*, size2, \
-                                                               size2, uint32_t, uint32_t, hipStream_t);        \
+                                                               size2, uint32_t, uint32_t, hipStream_t, uint32_t*); \
     template void launch_prepare_split<float, float, HT>(const float*, HT*, HT*, const float*, const float*,    \
                                                          const float*, const size2*, size2, size2, uint32_t,    \
-                                                         uint32_t, hipStream_t);                               \
+                                                         uint32_t, hipStream_t, uint32_t*);                    \
     template void launch_prepare_split<double, double, HT>(const double*, HT*, HT*, const float*, const float*, \
                                                            const double*, const size2*, size2, size2, uint32_t, \
-                                                           uint32_t, hipStream_t);
+                                                           uint32_t, hipStream_t, uint32_t*);
 INST_MFMA(__bf16)
 INST_MFMA(_Float16)
 

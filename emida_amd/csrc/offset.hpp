@@ -160,6 +160,7 @@ struct emida_offset_s {
     void* d_hyf = nullptr;
     void* d_pic_hi = nullptr;  // matrix-core policy: prepared pattern slices, hi / lo halves
     void* d_pic_lo = nullptr;
+    void* d_smax = nullptr;    // matrix-core policy with refinement: screen maximum per slice
 
     struct outset {
         void* maxes = nullptr;
@@ -198,12 +199,13 @@ struct emida_offset_s {
         void* corr = nullptr;
         void* pic_hi = nullptr;
         void* pic_lo = nullptr;
+        void* smax = nullptr;
     };
     static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
     size_t scratch_sizes(uint32_t n_pat, size_t* parts) const {
         const size_t tn = (size_t)nb * n_pat;
         parts[0] = al256(std::max<size_t>(tn, nb) * sum_bytes);
-        parts[1] = parts[2] = parts[3] = parts[4] = 0;
+        parts[1] = parts[2] = parts[3] = parts[4] = parts[5] = 0;
         if (policy == EMIDA_CROSS_POLICY_BRUTE) {
             parts[1] = al256(tn * slice.area() * t_bytes());
             parts[2] = al256(tn * cross.area() * t_bytes());
@@ -212,29 +214,30 @@ struct emida_offset_s {
             parts[2] = al256(tn * cross.area() * sizeof(float));
             parts[3] = al256(tn * slice.area() * 2);
             if (mfma_split(mfma_mode)) parts[4] = parts[3];
+            if (mfma_refine) parts[5] = al256(tn * sizeof(uint32_t));
         } else if (fft_padded(side) > 128) {
             parts[2] = al256(fft256_work_bytes(nb, (uint32_t)tn));  // column-stage work buffer
         }
-        return parts[0] + parts[1] + parts[2] + parts[3] + parts[4];
+        return parts[0] + parts[1] + parts[2] + parts[3] + parts[4] + parts[5];
     }
     size_t scratch_bytes(uint32_t n_pat) const {
-        size_t parts[5];
+        size_t parts[6];
         return scratch_sizes(n_pat, parts);
     }
     scratch_t carve(void* base, uint32_t n_pat) const {
-        size_t parts[5];
+        size_t parts[6];
         scratch_sizes(n_pat, parts);
         char* b = (char*)base;
-        void* ptr[5];
-        for (int i = 0; i < 5; ++i) {
+        void* ptr[6];
+        for (int i = 0; i < 6; ++i) {
             ptr[i] = parts[i] ? b : nullptr;
             b += parts[i];
         }
-        return {ptr[0], ptr[1], ptr[2], ptr[3], ptr[4]};
+        return {ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5]};
     }
     const void* float_hx() const { return d_hxf ? d_hxf : d_hx; }
     const void* float_hy() const { return d_hyf ? d_hyf : d_hy; }
-    scratch_t own_scratch() const { return {d_sums, d_slices, d_corr, d_pic_hi, d_pic_lo}; }
+    scratch_t own_scratch() const { return {d_sums, d_slices, d_corr, d_pic_hi, d_pic_lo, d_smax}; }
 
     size_t ti_bytes() const { return T == EMIDA_F32 ? 8 : 16; }
     size_t t_bytes() const { return T == EMIDA_F32 ? 4 : 8; }
@@ -406,6 +409,7 @@ struct emida_offset_s {
                 d_pic_hi = alloc(scratch_total * slice.area() * half);
                 if (mfma_split(mfma_mode)) d_pic_lo = alloc(scratch_total * slice.area() * half);
                 d_corr = alloc(scratch_total * cross.area() * sizeof(float));
+                if (mfma_refine) d_smax = alloc(scratch_total * sizeof(uint32_t));
             }
         } else {
             const size_t m = fft_padded(side);
@@ -459,13 +463,14 @@ struct emida_offset_s {
                         using HT = typename decltype(ht)::type;
                         launch_prepare_split<IT, ST, HT>(pics, (HT*)sc.pic_hi, (HT*)sc.pic_lo,
                                                          (const float*)float_hx(), (const float*)float_hy(), sums,
-                                                         d_begins, src, slice, nb, tn, st);
+                                                         d_begins, src, slice, nb, tn, st, (uint32_t*)sc.smax);
                         if (mfma_refine)
                             launch_prepare<IT, TT, ST>(pics, (TT*)sc.slices, (const TT*)d_hx, (const TT*)d_hy, sums,
                                                        d_begins, src, slice, slice, nb, tn, st);
                         if (timing) t0 = record_start(st);
                         launch_mfma_corr<HT>((const HT*)sc.pic_hi, (const HT*)sc.pic_lo, (const HT*)d_ref,
-                                             (const HT*)d_ref_lo, (float*)sc.corr, slice, cross, nb, n_pat, st);
+                                             (const HT*)d_ref_lo, (float*)sc.corr, slice, cross, nb, n_pat, st,
+                                             (uint32_t*)sc.smax);
                         if (timing) record_stop(st, t0);
                     });
                     if (mfma_refine) {
@@ -473,7 +478,8 @@ struct emida_offset_s {
                         mfma_refine_bounds(mfma_mode, slice, T == EMIDA_F64, tol_rel, tol_abs);
                         launch_refine<TT>((const float*)sc.corr, (size_t)tn * cross.area(), (const TT*)sc.slices,
                                           (const TT*)d_ref_t, slice, cross, nb, tol_rel, tol_abs, fit,
-                                          (data_index<TT>*)maxes, (TT*)neigh, offsets, coefs, tn, st);
+                                          (data_index<TT>*)maxes, (TT*)neigh, offsets, coefs, tn, st,
+                                          (const uint32_t*)sc.smax);
                     } else {
                         launch_peak<float>((const float*)sc.corr, (size_t)tn * cross.area(), cross.area(), cross,
                                            POLICY_BRUTE, fit, (data_index<float>*)maxes, (float*)neigh, offsets,

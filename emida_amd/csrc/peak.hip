@@ -380,7 +380,7 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
                                                     int fit, const double* __restrict__ lsq,
                                                     data_index<T>* __restrict__ maxes, T* __restrict__ neigh,
                                                     double* __restrict__ offsets, double* __restrict__ coefs,
-                                                    uint32_t total) {
+                                                    uint32_t total, const uint32_t* __restrict__ smax) {
     __shared__ float svf[NT / kWave];
     __shared__ uint32_t si[NT / kWave];
     __shared__ double sd[NT / kWave];
@@ -403,22 +403,29 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
     const T* ps = p + (size_t)sn * a;
     const T* rs = r + (size_t)(sn % nb) * a;
 
-    // screen maximum (NaN ignored) and its lowest index; 8 loads in flight per thread
+    // screen maximum (NaN ignored) and its lowest index; 8 loads in flight per thread.
+    // Given the maximum from the matrix-core kernel, the index is found by the
+    // candidate pass below instead (one read of the screen instead of two).
     constexpr int U = 8;
     float m = -__builtin_huge_valf();
     uint32_t mi = 0xffffffffu;
-    for (uint32_t i0 = tid; i0 < area; i0 += NT * U) {
-        float v[U];
+    if (smax) {
+        const uint32_t code = smax[sn];
+        if (code) m = screen_max_value(code);
+    } else {
+        for (uint32_t i0 = tid; i0 < area; i0 += NT * U) {
+            float v[U];
 #pragma unroll
-        for (int k = 0; k < U; ++k) v[k] = i0 + k * NT < area ? sc[i0 + k * NT] : -__builtin_huge_valf();
+            for (int k = 0; k < U; ++k) v[k] = i0 + k * NT < area ? sc[i0 + k * NT] : -__builtin_huge_valf();
 #pragma unroll
-        for (int k = 0; k < U; ++k)
-            if (v[k] > m) {
-                m = v[k];
-                mi = i0 + k * NT;
-            }
+            for (int k = 0; k < U; ++k)
+                if (v[k] > m) {
+                    m = v[k];
+                    mi = i0 + k * NT;
+                }
+        }
+        block_argmax<float, NT>(m, mi, svf, si);
     }
-    block_argmax<float, NT>(m, mi, svf, si);
 
     // ||p||, ||r|| in double
     double pp = 0, rr = 0;
@@ -455,17 +462,22 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
     if (tid == 0) ncand = 0;
     __syncthreads();
     const double thr = (double)m - tau;
+    uint32_t mil = 0xffffffffu;  // lowest index holding m (given maximum)
     for (uint32_t i0 = tid; i0 < area; i0 += NT * U) {
         float v[U];
 #pragma unroll
         for (int k = 0; k < U; ++k) v[k] = i0 + k * NT < area ? sc[i0 + k * NT] : -__builtin_huge_valf();
 #pragma unroll
-        for (int k = 0; k < U; ++k)
+        for (int k = 0; k < U; ++k) {
+            const uint32_t idx = i0 + k * NT;
             if ((double)v[k] >= thr) {
                 const uint32_t c = atomicAdd(&ncand, 1u);
-                if (c < kRefineCap) cand[c] = i0 + k * NT;
+                if (c < kRefineCap) cand[c] = idx;
             }
+            if (smax && idx < area && v[k] == m && idx < mil) mil = idx;
+        }
     }
+    if (smax) mi = block_min_u32<NT>(mil, si);
     __syncthreads();
     uint32_t nc = ncand;
     if (nc > (uint32_t)kRefineCap || mi == 0xffffffffu) {
@@ -597,20 +609,21 @@ void launch_subpixel(const T* neigh, int s, uint32_t n, const data_index<T>* max
 template <class T>
 void launch_refine(const float* screen, size_t corr_len, const T* p, const T* r, size2 s, size2 cross, uint32_t nb,
                    double tol_rel, double tol_abs, int fit, data_index<T>* maxes, T* neigh, double* offsets,
-                   double* coefs, uint32_t total, hipStream_t st) {
+                   double* coefs, uint32_t total, hipStream_t st, const uint32_t* screen_max) {
     if (total == 0) return;
     const double* lsq = lstsq_device_table() + lstsq_offset(fit);
     const size_t lds = (size_t)(2 * kBoxY + kBoxRows - 1) * box_stride(s.x) * sizeof(T);
     refine_kernel<T, kRefineNT><<<total, kRefineNT, lds, st>>>(screen, corr_len, p, r, s, cross, nb, tol_rel, tol_abs,
-                                                          fit, lsq, maxes, neigh, offsets, coefs, total);
+                                                          fit, lsq, maxes, neigh, offsets, coefs, total,
+                                                          screen_max);
     EMIDA_LAUNCHED("refine_kernel");
 }
 template void launch_refine<float>(const float*, size_t, const float*, const float*, size2, size2, uint32_t, double,
                                    double, int, data_index<float>*, float*, double*, double*, uint32_t,
-                                   hipStream_t);
+                                   hipStream_t, const uint32_t*);
 template void launch_refine<double>(const float*, size_t, const double*, const double*, size2, size2, uint32_t,
                                     double, double, int, data_index<double>*, double*, double*, double*, uint32_t,
-                                    hipStream_t);
+                                    hipStream_t, const uint32_t*);
 
 #define INST_PEAK(T)                                                                                          \
     template void launch_peak<T>(const T*, size_t, uint32_t, size2, int, int, data_index<T>*, T*, double*,     \

@@ -1,6 +1,8 @@
 """CPU-only checks of the HIP library: it loads, exports every declared entry point,
 and its host-side pieces (tables, windows, argument validation) agree with the
 oracle.  No GPU call is made here."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -103,3 +105,33 @@ def test_synthetic_patterns_are_deterministic():
     np.testing.assert_array_equal(a.pics, b.pics)
     np.testing.assert_array_equal(a.ref, b.ref)
     assert a.pics.dtype == np.uint16 and a.pics.shape == (2, 64, 64)
+
+
+def test_entry_points_validate_arguments_before_any_device_call():
+    # argument errors are reported without touching the GPU (this runs without one)
+    L = C.lib()
+    d = ctypes.c_void_p(1)  # never dereferenced: validation fails first
+    calib = C.Calib(436.0, 436.0, 436.0)
+
+    def err(rc):
+        return rc, C.lib().emida_last_error().decode()
+
+    fit = L.emida_fit_deformation
+    assert err(fit(None, 0, 10, None, None, 1, calib, 1e-6, 10, *[None] * 7)) == (C.ERR_ARG, "null argument")
+    for n in (0, 8193):
+        assert err(fit(d, 0, n, d, d, 1, calib, 1e-6, 10, *[None] * 7))[1] == "n_rois must be in 1..8192"
+    assert "more than 4 ROIs" in err(fit(d, 0, 4, d, None, 1, calib, 1e-6, 10, *[None] * 7))[1]
+    assert fit(d, 0, 10, d, d, 0, calib, 1e-6, 10, *[None] * 7) == C.OK  # no patterns: nothing to do
+
+    h = ctypes.c_void_p()
+    for cfg, msg in ((C.ServerConfig(0, 1, 1, 1, 1, 0), "must be positive"),
+                     (C.ServerConfig(1 << 20, 1, 17, 1, 1, 0), "at most 16 streams")):
+        rc, m = err(L.emida_server_create(ctypes.byref(h), ctypes.byref(cfg)))
+        assert rc == C.ERR_ARG and msg in m and not h.value
+
+    rc, m = err(L.emida_run_cross_corr_mfma(0, d, d, d, C.Size2(12, 12), C.Size2(23, 23), 1, 1, None))
+    assert rc == C.ERR_UNSUPPORTED and "multiple of 8" in m
+    rc, m = err(L.emida_run_cross_corr_mfma(0, d, d, d, C.Size2(16, 16), C.Size2(33, 31), 1, 1, None))
+    assert rc == C.ERR_ARG and "cross size" in m
+    rc, m = err(L.emida_run_cross_corr(1, d, d, d, C.Size2(4, 4), C.Size2(10, 9), 1, 1, None))
+    assert rc == C.ERR_ARG and "at most 2*size+1" in m

@@ -312,66 +312,97 @@ __device__ inline T exact_at(size_t g, size_t corr_len, const T* p, const T* r, 
 
 // Exact values of a box of shifts of one slice, all threads of the workgroup
 // together: thread t < bw * bh takes position (x0 + t % bw, y0 + t / bw) of the cross
-// buffer (shift = position - (cross - 1) / 2).  Chunks of kBoxY reference rows and the
-// pattern rows they meet are staged in LDS (row stride box_stride(s.x)); each thread
-// accumulates its own shift in the brute order (y outer, x inner, fma), loading 8
-// products' operands ahead of the 8 dependent fmas, so the values equal exact_corr's.
-// LDS: rrow kBoxY rows, prow kBoxY + kBoxRows - 1 rows.
+// buffer (shift = position - (cross - 1) / 2).  The lanes of a wave walk the union of
+// the box's overlaps in lockstep (y outer, x inner): chunks of kBoxY rows of the
+// pattern slice are staged in LDS with zeros wherever a shift leaves the slice (row
+// stride box_stride of the staged width), every lane takes the same x at each step,
+// and the reference value of that step is broadcast from the lane holding it (each
+// lane holds one or two elements of the current reference row, so slices are at most
+// 2 * kWave wide), so LDS serves one operand per product instead of two.  The terms
+// a lane adds outside its own overlap are exact zeros (fma(0, r, acc) == acc for
+// finite values, and the accumulator is never -0), so each value equals exact_corr's
+// bit for bit.  LDS: kBoxY + kBoxRows - 1 rows of box_stride(s.x + kBoxW - 1).
 constexpr int kBoxRows = 16;
+constexpr int kBoxW = 32;
 constexpr int kBoxY = 4;
 
-// LDS row stride (elements) of the staged rows: congruent to 9 mod 32, so the rows a
-// box reads at one step (consecutive shifts of up to 7 rows) start 18 banks apart
-// and a wave's 8-byte reads conflict at most 2-way (a stride of s.x + 1 put the rows
-// 2 banks apart: up to 7-way conflicts).
+// LDS row stride (elements) of the staged rows: the smallest stride >= sx congruent to
+// 9 mod 32, so the rows a box reads at one step (consecutive shifts of up to 7 rows)
+// start 18 banks apart and a wave's 8-byte reads conflict at most 2-way (a stride of
+// sx + 1 put the rows 2 banks apart: up to 7-way conflicts).
 __host__ __device__ constexpr int box_stride(int sx) { return sx + ((9 - sx % 32) + 32) % 32; }
+
+// The value lane l of the wave holds (l wave-uniform), as a scalar operand.
+__device__ inline double bcast(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline float bcast(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
 
 template <class T, int NT>
 __device__ inline void eval_box(const T* __restrict__ p, const T* __restrict__ r, size2 s, size2 cross, int x0,
-                                int y0, int bw, int bh, T* rrow, T* prow, T* out) {
+                                int y0, int bw, int bh, T* prow, T* out) {
     const int tid = threadIdx.x, S = (int)s.x, SY = (int)s.y;
-    const int PS = box_stride(S);
     const int rx = (int)(cross.x - 1) / 2, ry = (int)(cross.y - 1) / 2;
-    const int sy_lo = y0 - ry, sy_hi = y0 + bh - 1 - ry;  // shift range of the box rows
-    const bool mine = tid < bw * bh;
-    const int sx = x0 + (mine ? tid % bw : 0) - rx, sy = y0 + (mine ? tid / bw : 0) - ry;
-    const int xa = max(-sx, 0), xb = min(S - sx, S);
-    T acc = 0;
+    const int sx_lo = x0 - rx, sy_lo = y0 - ry;  // shift range of the box
+    const int sx_hi = sx_lo + bw - 1, sy_hi = sy_lo + bh - 1;
+    const int xlo = max(0, -sx_hi), xhi = min(S, S - sx_lo);  // union of the overlaps
     const int ylo = max(0, -sy_hi), yhi = min(SY, SY - sy_lo);
+    const int cw = xhi - xlo + bw - 1, c0 = xlo + sx_lo;  // staged columns, first one
+    const int PS = box_stride(cw);
+    const int n = bw * bh;
+    const bool wave_on = tid - tid % kWave < n;  // waves with no position skip the sums
+    const int t = min(tid, n - 1), lx = t % bw, ly = t / bw;
+    T acc = 0;
+    const int lane = tid % kWave, l0 = min(lane, S - 1), l1 = min(lane + kWave, S - 1);
+    T rn0 = 0, rn1 = 0;  // the next reference row, one element per lane (two for S > 64)
+    if (ylo < yhi) {
+        rn0 = r[(size_t)ylo * S + l0];
+        rn1 = r[(size_t)ylo * S + l1];
+    }
     for (int yc = ylo; yc < yhi; yc += kBoxY) {
         const int ny = min(kBoxY, yhi - yc);
         __syncthreads();
-        for (int e = tid; e < ny * S; e += NT) rrow[(e / S) * PS + e % S] = r[(size_t)(yc + e / S) * S + e % S];
-        for (int e = tid; e < (ny + bh - 1) * S; e += NT) {
-            const int Y = yc + sy_lo + e / S;
-            prow[(e / S) * PS + e % S] = (Y >= 0 && Y < SY) ? p[(size_t)Y * S + e % S] : T(0);
+        for (int e = tid; e < (ny + bh - 1) * cw; e += NT) {
+            const int j = e / cw, cc = e % cw, c = cc + c0, Y = yc + sy_lo + j;
+            prow[j * PS + cc] = (Y >= 0 && Y < SY && c >= 0 && c < S) ? p[(size_t)Y * S + c] : T(0);
         }
         __syncthreads();
-        if (!mine) continue;
+        if (!wave_on) continue;
         for (int k = 0; k < ny; ++k) {
-            const int y = yc + k;
-            if (y + sy < 0 || y + sy >= SY) continue;
-            const T* pr = prow + (size_t)(k + sy - sy_lo) * PS + sx;
-            const T* qr = rrow + (size_t)k * PS;
-            int x = xa;
-            for (; x + 8 <= xb; x += 8) {
-                T u[8], v[8];
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    u[i] = pr[x + i];
-                    v[i] = qr[x + i];
-                }
-#pragma unroll
-                for (int i = 0; i < 8; ++i) acc = fma(u[i], v[i], acc);
+            const T* pr = prow + (k + ly) * PS + lx - xlo;  // pr[x] = pattern (y + sy, x + sx)
+            const T r0 = rn0, r1 = rn1;
+            if (yc + k + 1 < yhi) {
+                rn0 = r[(size_t)(yc + k + 1) * S + l0];
+                rn1 = r[(size_t)(yc + k + 1) * S + l1];
             }
-            for (; x < xb; ++x) acc = fma(pr[x], qr[x], acc);
+            int x = xlo;
+            const int xm = min(xhi, kWave);  // x < kWave from r0, the rest from r1
+            // the pattern operands of 8 products are loaded before their 8 dependent fmas
+            for (; x + 8 <= xm; x += 8) {
+                T u[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) u[i] = pr[x + i];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) acc = fma(u[i], bcast(r0, x + i), acc);
+            }
+            for (; x < xm; ++x) acc = fma(pr[x], bcast(r0, x), acc);
+            for (; x + 8 <= xhi; x += 8) {
+                T u[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) u[i] = pr[x + i];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) acc = fma(u[i], bcast(r1, x + i - kWave), acc);
+            }
+            for (; x < xhi; ++x) acc = fma(pr[x], bcast(r1, x - kWave), acc);
         }
     }
-    if (mine) out[tid] = acc;
+    if (tid < n) out[tid] = acc;
     __syncthreads();
 }
-
-
 
 template <class T, int NT>
 __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ screen, size_t corr_len,
@@ -392,8 +423,7 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
     __shared__ T boxv[NT];
     __shared__ int bb[4];
     extern __shared__ __align__(16) unsigned char refine_lds[];
-    T* rrow = reinterpret_cast<T*>(refine_lds);  // kBoxY rows of box_stride(s.x)
-    T* prow = rrow + kBoxY * box_stride(s.x);     // kBoxY + kBoxRows - 1 rows of box_stride(s.x)
+    T* prow = reinterpret_cast<T*>(refine_lds);  // staged pattern rows of eval_box
     const uint32_t sn = blockIdx.x;
     if (sn >= total) return;
     const int tid = threadIdx.x;
@@ -508,10 +538,10 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
     const int cbw = bb[2] - bb[0] + 1, cbh = bb[3] - bb[1] + 1;
     const int ex0 = max(0, bb[0] - rad), ey0 = max(0, bb[1] - rad);
     const int ebw = min((int)cross.x - 1, bb[2] + rad) - ex0 + 1, ebh = min((int)cross.y - 1, bb[3] + rad) - ey0 + 1;
-    const bool merged = ebh <= kBoxRows && ebw * ebh <= NT;
-    const bool boxed = merged || (cbh <= kBoxRows && cbw * cbh <= NT);
+    const bool merged = ebh <= kBoxRows && ebw <= kBoxW && ebw * ebh <= NT;
+    const bool boxed = merged || (cbh <= kBoxRows && cbw <= kBoxW && cbw * cbh <= NT);
     const int bx0 = merged ? ex0 : bb[0], by0 = merged ? ey0 : bb[1], bw = merged ? ebw : cbw;
-    if (boxed) eval_box<T, NT>(ps, rs, s, cross, bx0, by0, bw, merged ? ebh : cbh, rrow, prow, boxv);
+    if (boxed) eval_box<T, NT>(ps, rs, s, cross, bx0, by0, bw, merged ? ebh : cbh, prow, boxv);
     T best = lowest<T>();
     uint32_t bi = 0xffffffffu;
     for (uint32_t c = tid; c < nc; c += NT) {
@@ -536,7 +566,7 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
         ny0 = max(0, (int)mp.y - rad);
         nbw = min((int)cross.x - 1, (int)mp.x + rad) - nx0 + 1;
         const int nbh = min((int)cross.y - 1, (int)mp.y + rad) - ny0 + 1;
-        eval_box<T, NT>(ps, rs, s, cross, nx0, ny0, nbw, nbh, rrow, prow, boxv);
+        eval_box<T, NT>(ps, rs, s, cross, nx0, ny0, nbw, nbh, prow, boxv);
     }
     for (int t = tid; t < k; t += NT) {
         const int i = t % fit, j = t / fit;
@@ -611,8 +641,9 @@ void launch_refine(const float* screen, size_t corr_len, const T* p, const T* r,
                    double tol_rel, double tol_abs, int fit, data_index<T>* maxes, T* neigh, double* offsets,
                    double* coefs, uint32_t total, hipStream_t st, const uint32_t* screen_max) {
     if (total == 0) return;
+    if (s.x > 2 * (uint32_t)kWave) throw arg_error("exact refinement needs slices at most 128 wide");
     const double* lsq = lstsq_device_table() + lstsq_offset(fit);
-    const size_t lds = (size_t)(2 * kBoxY + kBoxRows - 1) * box_stride(s.x) * sizeof(T);
+    const size_t lds = (size_t)(kBoxY + kBoxRows - 1) * box_stride((int)s.x + kBoxW - 1) * sizeof(T);
     refine_kernel<T, kRefineNT><<<total, kRefineNT, lds, st>>>(screen, corr_len, p, r, s, cross, nb, tol_rel, tol_abs,
                                                           fit, lsq, maxes, neigh, offsets, coefs, total,
                                                           screen_max);

@@ -315,21 +315,22 @@ __device__ inline T exact_at(size_t g, size_t corr_len, const T* p, const T* r, 
 // buffer (shift = position - (cross - 1) / 2).  The lanes of a wave walk the union of
 // the box's overlaps in lockstep (y outer, x inner): chunks of kBoxY rows of the
 // pattern slice are staged in LDS with zeros wherever a shift leaves the slice (row
-// stride box_stride of the staged width), every lane takes the same x at each step,
+// stride below), every lane takes the same x at each step,
 // and the reference value of that step is broadcast from the lane holding it (each
 // lane holds one or two elements of the current reference row, so slices are at most
 // 2 * kWave wide), so LDS serves one operand per product instead of two.  The terms
 // a lane adds outside its own overlap are exact zeros (fma(0, r, acc) == acc for
 // finite values, and the accumulator is never -0), so each value equals exact_corr's
-// bit for bit.  LDS: kBoxY + kBoxRows - 1 rows of box_stride(s.x + kBoxW - 1).
-constexpr int kBoxRows = 16;
+// bit for bit.  LDS: kBoxY + kBoxRows - 1 rows of box_stride(s.x + kBoxW - 1); boxes
+// of at most 12 rows keep it at 20 KB for 128-wide double slices, so six workgroups
+// share a CU (the register limit).
+constexpr int kBoxRows = 12;
 constexpr int kBoxW = 32;
 constexpr int kBoxY = 4;
 
-// LDS row stride (elements) of the staged rows: the smallest stride >= sx congruent to
-// 9 mod 32, so the rows a box reads at one step (consecutive shifts of up to 7 rows)
-// start 18 banks apart and a wave's 8-byte reads conflict at most 2-way (a stride of
-// sx + 1 put the rows 2 banks apart: up to 7-way conflicts).
+// Row stride of the allocation (elements): the smallest stride >= sx congruent to 9 mod
+// 32.  A box whose conflict-free stride does not fit it falls back to this stride, on
+// which consecutive box rows start 18 banks apart (at most 2-way conflicts).
 __host__ __device__ constexpr int box_stride(int sx) { return sx + ((9 - sx % 32) + 32) % 32; }
 
 // The value lane l of the wave holds (l wave-uniform), as a scalar operand.
@@ -352,7 +353,13 @@ __device__ inline void eval_box(const T* __restrict__ p, const T* __restrict__ r
     const int xlo = max(0, -sx_hi), xhi = min(S, S - sx_lo);  // union of the overlaps
     const int ylo = max(0, -sy_hi), yhi = min(SY, SY - sy_lo);
     const int cw = xhi - xlo + bw - 1, c0 = xlo + sx_lo;  // staged columns, first one
-    const int PS = box_stride(cw);
+    // a row stride congruent to bw modulo the 32 banks of 4 bytes makes the addresses of
+    // a step, row * PS + column for position t = row * bw + column, consecutive modulo
+    // the banks, so a step's reads are free of bank conflicts (refining 128x128 double
+    // slices, the fallback stride cost 4.9 conflict cycles per LDS instruction, this 0.02)
+    constexpr int kSweep = 128 / (int)sizeof(T);
+    const int PSc = cw + ((bw - cw) % kSweep + kSweep) % kSweep;
+    const int PS = PSc <= box_stride(S + kBoxW - 1) ? PSc : box_stride(cw);
     const int n = bw * bh;
     const bool wave_on = tid - tid % kWave < n;  // waves with no position skip the sums
     const int t = min(tid, n - 1), lx = t % bw, ly = t / bw;

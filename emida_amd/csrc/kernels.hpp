@@ -12,6 +12,15 @@ void launch_sum(const IN* pics, S* sums, const size2* begins, size2 src, size2 s
 template <class IN, class T, class S>
 void launch_prepare(const IN* pics, T* slices, const T* hx, const T* hy, const S* sums, const size2* begins,
                     size2 src, size2 slice, size2 out, uint32_t nb, uint32_t total, hipStream_t st);
+// Sums and preparation of uint16 slices in one kernel (sum_kernel + prepare_kernel +
+// prepare_split_kernel with identical results); slices (T, slice layout) and hi/lo
+// (half planes scaled by in_scale) are optional.  Returns false, launching nothing,
+// when a slice does not fit the LDS staging (kSumPrepareMaxLds bytes of pixels).
+constexpr size_t kSumPrepareMaxLds = 32768;
+template <class T, class HT>
+bool launch_sum_prepare(const uint16_t* pics, uint32_t* sums, T* slices, const T* hx, const T* hy, HT* hi, HT* lo,
+                        const float* hxf, const float* hyf, float in_scale, uint32_t* zero, const size2* begins,
+                        size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st);
 
 // ---- brute.hip
 // res[(i*ref_slices + j)][...] = correlation of pics slice (i, j) with ref slice j.
@@ -121,5 +130,8 @@ __host__ __device__ inline float screen_max_value(uint32_t c) {
 }
 template <class HT>
 void launch_split(const float* v, HT* hi, HT* lo, size_t n, hipStream_t st);
+// The factor prepare_split_kernel applies before the split (1 for bf16, 1/2 for fp16).
+template <class HT>
+float mfma_in_scale();
 
 }  // namespace emida_amd

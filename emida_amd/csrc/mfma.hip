@@ -67,15 +67,16 @@ __global__ __launch_bounds__(256) void prepare_split_kernel(const IN* __restrict
                                                             const size2* __restrict__ begins, size2 src, size2 slice,
                                                             uint32_t nb, uint32_t total, uint32_t* __restrict__ zero) {
 #pragma clang fp contract(off)
-    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t area = slice.area();
-    const size_t sn = tid / area;
-    if (sn >= total) return;
-    const uint32_t st = (uint32_t)(tid % area);
+    // blocks_per_slice blocks per slice: wave-uniform 32-bit slice index
+    const uint32_t area = slice.area(), bps = div_up(area, 256u);
+    const uint32_t sn = blockIdx.x / bps;
+    const uint32_t st = (blockIdx.x - sn * bps) * 256u + threadIdx.x;
+    if (sn >= total || st >= area) return;
+    const size_t tid = (size_t)sn * area + st;
     if (zero && st == 0) zero[sn] = 0u;
     const uint32_t x = st % slice.x, y = st / slice.x;
     const size2 b0 = begins[sn % nb];
-    float v = (float)pics[(sn / nb) * src.area() + (size_t)(b0.y + y) * src.x + b0.x + x];
+    float v = (float)pics[(size_t)(sn / nb) * src.area() + (size_t)(b0.y + y) * src.x + b0.x + x];
     v -= (float)sums[sn] / (float)area;
     v = v * hx[x] * hy[y];
     v *= mfma_vec<HT>::in;
@@ -428,13 +429,20 @@ template <class IN, class S, class HT>
 void launch_prepare_split(const IN* pics, HT* hi, HT* lo, const float* hx, const float* hy, const S* sums,
                           const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st,
                           uint32_t* zero) {
-    const size_t n = (size_t)slice.area() * total;
-    if (n == 0) return;
-    prepare_split_kernel<IN, S, HT><<<(unsigned)div_up(n, (size_t)256), 256, 0, st>>>(pics, hi, lo, hx, hy, sums,
-                                                                                     begins, src, slice, nb, total,
-                                                                                     zero);
+    const size_t blocks = (size_t)div_up(slice.area(), 256u) * total;
+    if (blocks == 0) return;
+    if (blocks > 0xffffffffu) throw arg_error("prepare: too many slices for one launch");
+    prepare_split_kernel<IN, S, HT><<<(unsigned)blocks, 256, 0, st>>>(pics, hi, lo, hx, hy, sums, begins, src, slice,
+                                                                     nb, total, zero);
     EMIDA_LAUNCHED("prepare_split_kernel");
 }
+
+template <class HT>
+float mfma_in_scale() {
+    return mfma_vec<HT>::in;
+}
+template float mfma_in_scale<__bf16>();
+template float mfma_in_scale<_Float16>();
 
 // Already prepared float slices -> hi (+ lo) halves.
 template <class HT>

@@ -431,6 +431,20 @@ struct emida_offset_s {
         allocated = true;
     }
 
+    // Sums and preparation of a batch in one kernel for uint16 patterns (uint32 sums);
+    // false when the input type or the slice size needs the separate kernels.
+    template <class IT, class ST, class TT, class HT>
+    bool fused_prepare(const IT* pics, ST* sums, TT* slices, HT* hi, HT* lo, uint32_t* zero, uint32_t total,
+                       hipStream_t st) {
+        if constexpr (std::is_same<IT, uint16_t>::value && std::is_same<ST, uint32_t>::value)
+            return launch_sum_prepare<TT, HT>(pics, sums, slices, (const TT*)d_hx, (const TT*)d_hy, hi, lo,
+                                              (const float*)float_hx(), (const float*)float_hy(),
+                                              hi ? mfma_in_scale<HT>() : 1.0f, zero, d_begins, src, slice, nb, total,
+                                              st);
+        else
+            return false;
+    }
+
     // ---- one batch on device data (src/gpu_offset.hpp:234-278)
     void run(const void* d_pics, uint32_t n_pat, void* maxes, void* neigh, double* offsets, double* coefs,
              hipStream_t st) {
@@ -448,9 +462,12 @@ struct emida_offset_s {
                 const IT* pics = (const IT*)d_pics;
                 ST* sums = (ST*)sc.sums;
                 if (policy == EMIDA_CROSS_POLICY_BRUTE) {
-                    launch_sum<IT, ST>(pics, sums, d_begins, src, slice, nb, tn, st);
-                    launch_prepare<IT, TT, ST>(pics, (TT*)sc.slices, (const TT*)d_hx, (const TT*)d_hy, sums, d_begins,
-                                               src, slice, slice, nb, tn, st);
+                    if (!fused_prepare<IT, ST, TT, _Float16>(pics, sums, (TT*)sc.slices, nullptr, nullptr, nullptr, tn,
+                                                             st)) {
+                        launch_sum<IT, ST>(pics, sums, d_begins, src, slice, nb, tn, st);
+                        launch_prepare<IT, TT, ST>(pics, (TT*)sc.slices, (const TT*)d_hx, (const TT*)d_hy, sums,
+                                                   d_begins, src, slice, slice, nb, tn, st);
+                    }
                     if (timing) t0 = record_start(st);
                     launch_cross_corr<TT>((const TT*)sc.slices, (const TT*)d_ref, (TT*)sc.corr, slice, cross, nb, n_pat,
                                           st);
@@ -458,15 +475,19 @@ struct emida_offset_s {
                     launch_peak<TT>((const TT*)sc.corr, (size_t)tn * cross.area(), cross.area(), cross, POLICY_BRUTE,
                                     fit, (data_index<TT>*)maxes, (TT*)neigh, offsets, coefs, tn, st);
                 } else if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA) {
-                    launch_sum<IT, ST>(pics, sums, d_begins, src, slice, nb, tn, st);
                     with_HT(mfma_mode, [&](auto ht) {
                         using HT = typename decltype(ht)::type;
-                        launch_prepare_split<IT, ST, HT>(pics, (HT*)sc.pic_hi, (HT*)sc.pic_lo,
-                                                         (const float*)float_hx(), (const float*)float_hy(), sums,
-                                                         d_begins, src, slice, nb, tn, st, (uint32_t*)sc.smax);
-                        if (mfma_refine)
-                            launch_prepare<IT, TT, ST>(pics, (TT*)sc.slices, (const TT*)d_hx, (const TT*)d_hy, sums,
-                                                       d_begins, src, slice, slice, nb, tn, st);
+                        if (!fused_prepare<IT, ST, TT, HT>(pics, sums, mfma_refine ? (TT*)sc.slices : nullptr,
+                                                           (HT*)sc.pic_hi, (HT*)sc.pic_lo, (uint32_t*)sc.smax, tn,
+                                                           st)) {
+                            launch_sum<IT, ST>(pics, sums, d_begins, src, slice, nb, tn, st);
+                            launch_prepare_split<IT, ST, HT>(pics, (HT*)sc.pic_hi, (HT*)sc.pic_lo,
+                                                             (const float*)float_hx(), (const float*)float_hy(), sums,
+                                                             d_begins, src, slice, nb, tn, st, (uint32_t*)sc.smax);
+                            if (mfma_refine)
+                                launch_prepare<IT, TT, ST>(pics, (TT*)sc.slices, (const TT*)d_hx, (const TT*)d_hy,
+                                                           sums, d_begins, src, slice, slice, nb, tn, st);
+                        }
                         if (timing) t0 = record_start(st);
                         launch_mfma_corr<HT>((const HT*)sc.pic_hi, (const HT*)sc.pic_lo, (const HT*)d_ref,
                                              (const HT*)d_ref_lo, (float*)sc.corr, slice, cross, nb, n_pat, st,

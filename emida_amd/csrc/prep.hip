@@ -61,17 +61,150 @@ __global__ __launch_bounds__(256) void prepare_kernel(const IN* __restrict__ pic
                                                       size2 src, size2 slice, size2 out, uint32_t nb,
                                                       uint32_t total) {
 #pragma clang fp contract(off)
-    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t area = slice.area();
-    const size_t sn = tid / area;
-    if (sn >= total) return;
-    const uint32_t st = (uint32_t)(tid % area);
+    // blocks_per_slice blocks per slice: the slice index is wave-uniform 32-bit
+    // arithmetic (no per-pixel 64-bit division)
+    const uint32_t area = slice.area(), bps = div_up(area, 256u);
+    const uint32_t sn = blockIdx.x / bps;
+    const uint32_t st = (blockIdx.x - sn * bps) * 256u + threadIdx.x;
+    if (sn >= total || st >= area) return;
     const uint32_t x = st % slice.x, y = st / slice.x;
     const size2 b0 = begins[sn % nb];
-    T pixel = (T)pics[(sn / nb) * src.area() + (size_t)(b0.y + y) * src.x + b0.x + x];
+    T pixel = (T)pics[(size_t)(sn / nb) * src.area() + (size_t)(b0.y + y) * src.x + b0.x + x];
     pixel -= (T)sums[sn] / (T)area;
     pixel = pixel * hx[x] * hy[y];
-    slices[sn * out.area() + (size_t)y * out.x + x] = pixel;
+    slices[(size_t)sn * out.area() + (size_t)y * out.x + x] = pixel;
+}
+
+// uint16 slices in one pass: one workgroup per slice stages the slice's pixels in LDS
+// and sums them in uint32 (exact, so the order does not matter: the sum equals
+// sum_kernel's), then writes the prepared T slice with prepare_kernel's arithmetic and
+// the hi/lo halves of the float prepared value with prepare_split_kernel's (each
+// output optional).  The pixels are read from HBM once instead of two or three times.
+template <class T, class HT>
+__global__ __launch_bounds__(256) void sum_prepare_kernel(const uint16_t* __restrict__ pics,
+                                                          uint32_t* __restrict__ sums, T* __restrict__ slices,
+                                                          const T* __restrict__ hx, const T* __restrict__ hy,
+                                                          HT* __restrict__ hi, HT* __restrict__ lo,
+                                                          const float* __restrict__ hxf, const float* __restrict__ hyf,
+                                                          float in_scale, uint32_t* __restrict__ zero,
+                                                          const size2* __restrict__ begins, size2 src, size2 slice,
+                                                          uint32_t nb, uint32_t total) {
+#pragma clang fp contract(off)
+    constexpr uint32_t NT = 256;
+    extern __shared__ __align__(16) uint16_t spx[];  // the slice's pixels, row-major
+    __shared__ uint32_t part[NT / kWave];
+    const uint32_t sn = blockIdx.x;
+    if (sn >= total) return;
+    const uint32_t area = slice.area(), sx = slice.x, tid = threadIdx.x;
+    const size2 b0 = begins[sn % nb];
+    const uint16_t* pic = pics + (size_t)(sn / nb) * src.area() + (size_t)b0.y * src.x + b0.x;
+    uint32_t s = 0;
+    constexpr uint32_t U = 16;  // loads in flight per thread
+    for (uint32_t e0 = tid; e0 < area; e0 += NT * U) {
+        uint16_t v[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k) {
+            const uint32_t e = e0 + k * NT;
+            v[k] = e < area ? pic[(size_t)(e / sx) * src.x + e % sx] : (uint16_t)0;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k) {
+            if (e0 + k * NT < area) spx[e0 + k * NT] = v[k];
+            s += v[k];
+        }
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) s += (uint32_t)__shfl_xor((int)s, off);
+    if (tid % kWave == 0) part[tid / kWave] = s;
+    __syncthreads();
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < NT / kWave; ++k) sum += part[k];
+    if (tid == 0) {
+        if (sums) sums[sn] = sum;
+        if (zero) zero[sn] = 0u;
+    }
+    // rows of a multiple of 4 pixels: 4 consecutive pixels of one row per thread and
+    // step (one 8-byte LDS read, vector table loads and stores); others one pixel
+    const bool quad = sx % 4 == 0;
+    const uint32_t step = quad ? 4 : 1;
+    if (slices) {
+        T* out = slices + (size_t)sn * area;
+        const T mean = (T)sum / (T)area;
+        for (uint32_t e = tid * step; e < area; e += NT * step) {
+            const uint32_t x = e % sx, y = e / sx;
+            if (quad) {
+                typedef T t4 __attribute__((ext_vector_type(4)));
+                typedef uint16_t u4 __attribute__((ext_vector_type(4)));
+                const u4 q = *reinterpret_cast<const u4*>(spx + e);
+                const t4 wx = *reinterpret_cast<const t4*>(hx + x);
+                const T wy = hy[y];
+                t4 v;
+                v.x = ((T)q.x - mean) * wx.x * wy;
+                v.y = ((T)q.y - mean) * wx.y * wy;
+                v.z = ((T)q.z - mean) * wx.z * wy;
+                v.w = ((T)q.w - mean) * wx.w * wy;
+                *reinterpret_cast<t4*>(out + e) = v;
+            } else {
+                T v = (T)spx[e];
+                v -= mean;
+                v = v * hx[x] * hy[y];
+                out[e] = v;
+            }
+        }
+    }
+    if (hi) {
+        const size_t o = (size_t)sn * area;
+        const float mean = (float)sum / (float)area;
+        for (uint32_t e = tid * step; e < area; e += NT * step) {
+            const uint32_t x = e % sx, y = e / sx;
+            if (quad) {
+                typedef float f4 __attribute__((ext_vector_type(4)));
+                typedef HT h4 __attribute__((ext_vector_type(4)));
+                typedef uint16_t u4 __attribute__((ext_vector_type(4)));
+                const u4 q = *reinterpret_cast<const u4*>(spx + e);
+                const f4 wx = *reinterpret_cast<const f4*>(hxf + x);
+                const float wy = hyf[y];
+                f4 v;
+                v.x = ((float)q.x - mean) * wx.x * wy * in_scale;
+                v.y = ((float)q.y - mean) * wx.y * wy * in_scale;
+                v.z = ((float)q.z - mean) * wx.z * wy * in_scale;
+                v.w = ((float)q.w - mean) * wx.w * wy * in_scale;
+                h4 h, l;
+                h.x = (HT)v.x;
+                h.y = (HT)v.y;
+                h.z = (HT)v.z;
+                h.w = (HT)v.w;
+                *reinterpret_cast<h4*>(hi + o + e) = h;
+                if (lo) {
+                    l.x = (HT)(v.x - (float)h.x);
+                    l.y = (HT)(v.y - (float)h.y);
+                    l.z = (HT)(v.z - (float)h.z);
+                    l.w = (HT)(v.w - (float)h.w);
+                    *reinterpret_cast<h4*>(lo + o + e) = l;
+                }
+            } else {
+                float v = (float)spx[e];
+                v -= mean;
+                v = v * hxf[x] * hyf[y];
+                v *= in_scale;
+                const HT h = (HT)v;
+                hi[o + e] = h;
+                if (lo) lo[o + e] = (HT)(v - (float)h);
+            }
+        }
+    }
+}
+
+template <class T, class HT>
+bool launch_sum_prepare(const uint16_t* pics, uint32_t* sums, T* slices, const T* hx, const T* hy, HT* hi, HT* lo,
+                        const float* hxf, const float* hyf, float in_scale, uint32_t* zero, const size2* begins,
+                        size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st) {
+    const size_t lds = (size_t)slice.area() * sizeof(uint16_t);
+    if (lds > kSumPrepareMaxLds) return false;
+    if (total == 0) return true;
+    sum_prepare_kernel<T, HT><<<total, 256, lds, st>>>(pics, sums, slices, hx, hy, hi, lo, hxf, hyf, in_scale, zero,
+                                                       begins, src, slice, nb, total);
+    EMIDA_LAUNCHED("sum_prepare_kernel");
+    return true;
 }
 
 template <class IN, class S>
@@ -85,12 +218,22 @@ void launch_sum(const IN* pics, S* sums, const size2* begins, size2 src, size2 s
 template <class IN, class T, class S>
 void launch_prepare(const IN* pics, T* slices, const T* hx, const T* hy, const S* sums, const size2* begins,
                     size2 src, size2 slice, size2 out, uint32_t nb, uint32_t total, hipStream_t st) {
-    const size_t n = (size_t)slice.area() * total;
-    if (n == 0) return;
-    prepare_kernel<IN, T, S><<<(unsigned)div_up(n, (size_t)256), 256, 0, st>>>(pics, slices, hx, hy, sums, begins,
-                                                                              src, slice, out, nb, total);
+    const size_t blocks = (size_t)div_up(slice.area(), 256u) * total;
+    if (blocks == 0) return;
+    if (blocks > 0xffffffffu) throw arg_error("prepare: too many slices for one launch");
+    prepare_kernel<IN, T, S><<<(unsigned)blocks, 256, 0, st>>>(pics, slices, hx, hy, sums, begins, src, slice, out,
+                                                               nb, total);
     EMIDA_LAUNCHED("prepare_kernel");
 }
+
+#define INST_SUM_PREP(T, HT)                                                                              \
+    template bool launch_sum_prepare<T, HT>(const uint16_t*, uint32_t*, T*, const T*, const T*, HT*, HT*,   \
+                                            const float*, const float*, float, uint32_t*, const size2*, size2, \
+                                            size2, uint32_t, uint32_t, hipStream_t);
+INST_SUM_PREP(double, _Float16)
+INST_SUM_PREP(double, __bf16)
+INST_SUM_PREP(float, _Float16)
+INST_SUM_PREP(float, __bf16)
 
 #define INST_SUM(IN, S)                                                                                  \
     template void launch_sum<IN, S>(const IN*, S*, const size2*, size2, size2, uint32_t, uint32_t, hipStream_t);

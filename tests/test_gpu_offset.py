@@ -113,6 +113,22 @@ def test_brute_bitexact_vs_oracle(T, fit):
     np.testing.assert_array_equal(offs, eo)
 
 
+@pytest.mark.parametrize("side", [128, 136])
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+def test_brute_both_preparation_paths_vs_oracle(T, side):
+    # uint16 slices of up to 128 x 128 (32 KB of pixels) take the one-kernel sums and
+    # preparation, larger ones the separate sum and prepare kernels
+    ps = _patterns(256, 2)
+    begins = synth.grid_begins(256, 2, side // 2)
+    offs, coefs, maxes, neigh = _run(T, BRUTE, ps, begins, (side, side), (9, 9), 3, 2)
+    eo, ec, em, en = O.get_offset(ps.pics.ravel(), ps.ref, ps.size, begins, (side, side), (9, 9), 2, 3, BRUTE, T,
+                                  return_intermediates=True)
+    np.testing.assert_array_equal(maxes["index"], em["index"])
+    np.testing.assert_array_equal(maxes["data"], em["data"])
+    np.testing.assert_array_equal(neigh, en)
+    np.testing.assert_array_equal(offs, eo)
+
+
 @pytest.mark.parametrize("cross", [(9, 9), (5, 7), (3, 3)])
 def test_brute_cropped_cross_with_edge_peaks_vs_oracle(cross):
     # displacements of +-3 px with a small correlation window put maxima on the

@@ -321,10 +321,10 @@ __device__ inline T exact_at(size_t g, size_t corr_len, const T* p, const T* r, 
 // 2 * kWave wide), so LDS serves one operand per product instead of two.  The terms
 // a lane adds outside its own overlap are exact zeros (fma(0, r, acc) == acc for
 // finite values, and the accumulator is never -0), so each value equals exact_corr's
-// bit for bit.  LDS: kBoxY + kBoxRows - 1 rows of box_stride(s.x + kBoxW - 1); boxes
-// of at most 12 rows keep it at 20 KB for 128-wide double slices, so six workgroups
-// share a CU (the register limit).
-constexpr int kBoxRows = 12;
+// bit for bit.  LDS: kBoxY + kBoxRows - 1 rows of box_stride(s.x + kBoxW - 1), which
+// also holds the box values afterwards; boxes of at most 14 rows keep it at 23 KB for
+// 128-wide double slices, so six workgroups share a CU (the register limit).
+constexpr int kBoxRows = 14;
 constexpr int kBoxW = 32;
 constexpr int kBoxY = 4;
 
@@ -407,6 +407,7 @@ __device__ inline void eval_box(const T* __restrict__ p, const T* __restrict__ r
             for (; x < xhi; ++x) acc = fma(pr[x], bcast(r1, x - kWave), acc);
         }
     }
+    __syncthreads();  // out may alias the staged rows
     if (tid < n) out[tid] = acc;
     __syncthreads();
 }
@@ -427,10 +428,10 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
     __shared__ uint32_t ncand;
     __shared__ T nbv[81];
     __shared__ double cf[6];
-    __shared__ T boxv[NT];
     __shared__ int bb[4];
     extern __shared__ __align__(16) unsigned char refine_lds[];
     T* prow = reinterpret_cast<T*>(refine_lds);  // staged pattern rows of eval_box
+    T* boxv = prow;  // the box values eval_box leaves (over the staged rows)
     const uint32_t sn = blockIdx.x;
     if (sn >= total) return;
     const int tid = threadIdx.x;

@@ -14,13 +14,14 @@ void launch_prepare(const IN* pics, T* slices, const T* hx, const T* hy, const S
                     size2 src, size2 slice, size2 out, uint32_t nb, uint32_t total, hipStream_t st);
 // Sums and preparation of uint16 slices in one kernel (sum_kernel + prepare_kernel +
 // prepare_split_kernel with identical results); slices (T, slice layout) and hi/lo
-// (half planes scaled by in_scale) are optional.  Returns false, launching nothing,
-// when a slice does not fit the LDS staging (kSumPrepareMaxLds bytes of pixels).
+// (half planes scaled by in_scale) are optional; zero (optional) gets zrow zeros per
+// slice.  Returns false, launching nothing, when a slice does not fit the LDS staging
+// (kSumPrepareMaxLds bytes of pixels).
 constexpr size_t kSumPrepareMaxLds = 32768;
 template <class T, class HT>
 bool launch_sum_prepare(const uint16_t* pics, uint32_t* sums, T* slices, const T* hx, const T* hy, HT* hi, HT* lo,
-                        const float* hxf, const float* hyf, float in_scale, uint32_t* zero, const size2* begins,
-                        size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st);
+                        const float* hxf, const float* hyf, float in_scale, uint32_t* zero, uint32_t zrow,
+                        const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st);
 
 // ---- brute.hip
 // res[(i*ref_slices + j)][...] = correlation of pics slice (i, j) with ref slice j.
@@ -102,18 +103,18 @@ void launch_hadamard(cplx<T>* pics, const cplx<T>* ref, size2 one, uint32_t ref_
 // ---- mfma.hip (matrix-core brute correlation, float results)
 // Prepared slices as half-type hi (+ lo when lo != nullptr) planes [slice][y][x];
 // corr: [pattern * nb + roi][cross.y][cross.x].  screen_max (optional, [pattern * nb +
-// roi], zero on entry): receives the largest value of each slice (NaN ignored) as an
-// ordered code, atomically.
+// roi][cross.y], zero on entry): receives the largest value of each row of each slice
+// (NaN ignored) as an ordered code, atomically.
 template <class HT>
 void launch_mfma_corr(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
                       uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* screen_max = nullptr);
 void check_mfma_shape(size2 s, size2 cross);
-// zero (optional): one uint32 per slice set to 0 (the screen maxima of the next
+// zero (optional): zrow uint32 per slice set to 0 (the screen row maxima of the next
 // launch_mfma_corr).
 template <class IN, class S, class HT>
 void launch_prepare_split(const IN* pics, HT* hi, HT* lo, const float* hx, const float* hy, const S* sums,
                           const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st,
-                          uint32_t* zero = nullptr);
+                          uint32_t* zero = nullptr, uint32_t zrow = 0);
 
 // Order-preserving uint32 code of a float (a < b  <=>  code(a) < code(b), -0 < +0);
 // 0 is below every code and stands for "no value".

@@ -65,7 +65,8 @@ __global__ __launch_bounds__(256) void prepare_split_kernel(const IN* __restrict
                                                             HT* __restrict__ lo, const float* __restrict__ hx,
                                                             const float* __restrict__ hy, const S* __restrict__ sums,
                                                             const size2* __restrict__ begins, size2 src, size2 slice,
-                                                            uint32_t nb, uint32_t total, uint32_t* __restrict__ zero) {
+                                                            uint32_t nb, uint32_t total, uint32_t* __restrict__ zero,
+                                                            uint32_t zrow) {
 #pragma clang fp contract(off)
     // blocks_per_slice blocks per slice: wave-uniform 32-bit slice index
     const uint32_t area = slice.area(), bps = div_up(area, 256u);
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(256) void prepare_split_kernel(const IN* __restrict
     const uint32_t st = (blockIdx.x - sn * bps) * 256u + threadIdx.x;
     if (sn >= total || st >= area) return;
     const size_t tid = (size_t)sn * area + st;
-    if (zero && st == 0) zero[sn] = 0u;
+    if (zero && st < zrow) zero[(size_t)sn * zrow + st] = 0u;
     const uint32_t x = st % slice.x, y = st / slice.x;
     const size2 b0 = begins[sn % nb];
     float v = (float)pics[(size_t)(sn / nb) * src.area() + (size_t)(b0.y + y) * src.x + b0.x + x];
@@ -293,41 +294,42 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
         }
         __syncthreads();
     }
-    // per-slice maximum of the values written below (NaN never wins a strict compare):
-    // per accumulator row j the lane's largest value over its y-shifts and tiles, then
-    // over the 32 lanes of the half (xor < 32 stays in the half); lane j < 16 of each
-    // half publishes pattern row 8 (j/4) + 4 (l/32) + j%4 with one atomic max
+    // per-row maxima of the values written below (NaN never wins a strict compare): for
+    // each y-shift, per accumulator row j the lane's largest value over its two tiles,
+    // then over the 32 lanes of the half (xor < 32 stays in the half); lane j < 16 of
+    // each half publishes pattern row 8 (j/4) + 4 (l/32) + j%4 with one atomic max
     if (smax) {
-        float mx[16];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) mx[j] = -__builtin_huge_valf();
 #pragma unroll
         for (int t = 0; t < TS; ++t) {
-            const bool yin = syw + t + ry < (int)cross.y;
+            const int row = syw + t + ry;
+            if (row >= (int)cross.y) continue;
+            float mx[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) mx[j] = -__builtin_huge_valf();
 #pragma unroll
             for (int u = 0; u < kMfmaTilesPerWave; ++u) {
-                const bool in = yin && sx0[u] + m_a + rx < (int)cross.x;
+                const bool in = sx0[u] + m_a + rx < (int)cross.x;
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
                     const float v = acc[t][u][j] * mfma_vec<HT>::out;
                     if (in && v > mx[j]) mx[j] = v;
                 }
             }
-        }
 #pragma unroll
-        for (int j = 0; j < 16; ++j)
+            for (int j = 0; j < 16; ++j)
 #pragma unroll
-            for (int off = 16; off > 0; off >>= 1) {
-                const float o = __shfl_xor(mx[j], off);
-                if (o > mx[j]) mx[j] = o;
+                for (int off = 16; off > 0; off >>= 1) {
+                    const float o = __shfl_xor(mx[j], off);
+                    if (o > mx[j]) mx[j] = o;
+                }
+            float mine = -__builtin_huge_valf();
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (j == m_a) mine = mx[j];
+            if (m_a < 16 && mine > -__builtin_huge_valf()) {
+                const uint32_t p = p0 + 8 * (m_a / 4) + 4 * half + (m_a % 4);
+                if (p < npat) atomicMax(&smax[((size_t)p * nb + roi) * cross.y + row], screen_max_code(mine));
             }
-        float mine = -__builtin_huge_valf();
-#pragma unroll
-        for (int j = 0; j < 16; ++j)
-            if (j == m_a) mine = mx[j];
-        if (m_a < 16 && mine > -__builtin_huge_valf()) {
-            const uint32_t p = p0 + 8 * (m_a / 4) + 4 * half + (m_a % 4);
-            if (p < npat) atomicMax(&smax[(size_t)p * nb + roi], screen_max_code(mine));
         }
     }
     // write the accumulators: D[8 (j/4) + 4 (l/32) + j%4][l%32]
@@ -428,12 +430,13 @@ void mfma_refine_bounds(int mode, size2 s, bool t_double, double& tol_rel, doubl
 template <class IN, class S, class HT>
 void launch_prepare_split(const IN* pics, HT* hi, HT* lo, const float* hx, const float* hy, const S* sums,
                           const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st,
-                          uint32_t* zero) {
+                          uint32_t* zero, uint32_t zrow) {
     const size_t blocks = (size_t)div_up(slice.area(), 256u) * total;
+    if (zero && zrow > slice.area()) throw arg_error("prepare: more screen rows than slice pixels");
     if (blocks == 0) return;
     if (blocks > 0xffffffffu) throw arg_error("prepare: too many slices for one launch");
     prepare_split_kernel<IN, S, HT><<<(unsigned)blocks, 256, 0, st>>>(pics, hi, lo, hx, hy, sums, begins, src, slice,
-                                                                     nb, total, zero);
+                                                                     nb, total, zero, zrow);
     EMIDA_LAUNCHED("prepare_split_kernel");
 }
 
@@ -468,13 +471,14 @@ void launch_split(const float* v, HT* hi, HT* lo, size_t n, hipStream_t st) {
     template void launch_split<HT>(const float*, HT*, HT*, size_t, hipStream_t);                               \
     template void launch_prepare_split<uint16_t, uint32_t, HT>(const uint16_t*, HT*, HT*, const float*,        \
                                                                const float*, const uint32_t*, const size2*, size2, \
-                                                               size2, uint32_t, uint32_t, hipStream_t, uint32_t*); \
+                                                               size2, uint32_t, uint32_t, hipStream_t, uint32_t*,  \
+                                                               uint32_t);                                          \
     template void launch_prepare_split<float, float, HT>(const float*, HT*, HT*, const float*, const float*,    \
                                                          const float*, const size2*, size2, size2, uint32_t,    \
-                                                         uint32_t, hipStream_t, uint32_t*);                    \
+                                                         uint32_t, hipStream_t, uint32_t*, uint32_t);          \
     template void launch_prepare_split<double, double, HT>(const double*, HT*, HT*, const float*, const float*, \
                                                            const double*, const size2*, size2, size2, uint32_t, \
-                                                           uint32_t, hipStream_t, uint32_t*);
+                                                           uint32_t, hipStream_t, uint32_t*, uint32_t);
 INST_MFMA(__bf16)
 INST_MFMA(_Float16)
 

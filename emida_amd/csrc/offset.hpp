@@ -160,7 +160,7 @@ struct emida_offset_s {
     void* d_hyf = nullptr;
     void* d_pic_hi = nullptr;  // matrix-core policy: prepared pattern slices, hi / lo halves
     void* d_pic_lo = nullptr;
-    void* d_smax = nullptr;    // matrix-core policy with refinement: screen maximum per slice
+    void* d_smax = nullptr;    // matrix-core policy with refinement: screen maximum per slice row
 
     struct outset {
         void* maxes = nullptr;
@@ -214,7 +214,7 @@ struct emida_offset_s {
             parts[2] = al256(tn * cross.area() * sizeof(float));
             parts[3] = al256(tn * slice.area() * 2);
             if (mfma_split(mfma_mode)) parts[4] = parts[3];
-            if (mfma_refine) parts[5] = al256(tn * sizeof(uint32_t));
+            if (mfma_refine) parts[5] = al256(tn * cross.y * sizeof(uint32_t));
         } else if (fft_padded(side) > 128) {
             parts[2] = al256(fft256_work_bytes(nb, (uint32_t)tn));  // column-stage work buffer
         }
@@ -409,7 +409,7 @@ struct emida_offset_s {
                 d_pic_hi = alloc(scratch_total * slice.area() * half);
                 if (mfma_split(mfma_mode)) d_pic_lo = alloc(scratch_total * slice.area() * half);
                 d_corr = alloc(scratch_total * cross.area() * sizeof(float));
-                if (mfma_refine) d_smax = alloc(scratch_total * sizeof(uint32_t));
+                if (mfma_refine) d_smax = alloc(scratch_total * cross.y * sizeof(uint32_t));
             }
         } else {
             const size_t m = fft_padded(side);
@@ -439,8 +439,8 @@ struct emida_offset_s {
         if constexpr (std::is_same<IT, uint16_t>::value && std::is_same<ST, uint32_t>::value)
             return launch_sum_prepare<TT, HT>(pics, sums, slices, (const TT*)d_hx, (const TT*)d_hy, hi, lo,
                                               (const float*)float_hx(), (const float*)float_hy(),
-                                              hi ? mfma_in_scale<HT>() : 1.0f, zero, d_begins, src, slice, nb, total,
-                                              st);
+                                              hi ? mfma_in_scale<HT>() : 1.0f, zero, cross.y, d_begins, src, slice,
+                                              nb, total, st);
         else
             return false;
     }
@@ -483,7 +483,8 @@ struct emida_offset_s {
                             launch_sum<IT, ST>(pics, sums, d_begins, src, slice, nb, tn, st);
                             launch_prepare_split<IT, ST, HT>(pics, (HT*)sc.pic_hi, (HT*)sc.pic_lo,
                                                              (const float*)float_hx(), (const float*)float_hy(), sums,
-                                                             d_begins, src, slice, nb, tn, st, (uint32_t*)sc.smax);
+                                                             d_begins, src, slice, nb, tn, st, (uint32_t*)sc.smax,
+                                                             cross.y);
                             if (mfma_refine)
                                 launch_prepare<IT, TT, ST>(pics, (TT*)sc.slices, (const TT*)d_hx, (const TT*)d_hy,
                                                            sums, d_begins, src, slice, slice, nb, tn, st);

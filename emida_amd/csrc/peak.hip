@@ -270,6 +270,7 @@ __global__ void subpixel_kernel(const T* __restrict__ neigh, int s, uint32_t n, 
 // A slice with more than kRefineCap candidates (a flat, noise-dominated screen) keeps
 // the screen's arg-max.
 constexpr int kRefineCap = 512;
+constexpr int kRowList = 256;  // screens of at most this many rows use the row maxima
 constexpr int kRefineNT = 256;
 
 template <class T>
@@ -425,7 +426,8 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
     __shared__ double sd[NT / kWave];
     __shared__ T sv[NT / kWave];
     __shared__ uint32_t cand[kRefineCap];
-    __shared__ uint32_t ncand;
+    __shared__ uint32_t ncand, nrow;
+    __shared__ uint16_t rlist[kRowList];  // rows of the screen that can hold candidates
     __shared__ T nbv[81];
     __shared__ double cf[6];
     __shared__ int bb[4];
@@ -447,9 +449,11 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
     constexpr int U = 8;
     float m = -__builtin_huge_valf();
     uint32_t mi = 0xffffffffu;
+    uint32_t mcode = 0;  // largest row-maximum code (0: no value)
     if (smax) {
-        const uint32_t code = smax[sn];
-        if (code) m = screen_max_value(code);
+        for (uint32_t r = tid; r < cross.y; r += NT) mcode = max(mcode, smax[(size_t)sn * cross.y + r]);
+        mcode = ~block_min_u32<NT>(~mcode, si);
+        if (mcode) m = screen_max_value(mcode);
     } else {
         for (uint32_t i0 = tid; i0 < area; i0 += NT * U) {
             float v[U];
@@ -496,23 +500,45 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
     const double np = sqrt(pp), nr = sqrt(rr);
     const double tau = tol_rel * np * nr + tol_abs * (np + nr);
 
-    // candidates
-    if (tid == 0) ncand = 0;
+    // candidates.  Given the row maxima, only rows whose maximum reaches the threshold
+    // (or equals the slice maximum) can hold one, and only those rows are read.
+    if (tid == 0) ncand = nrow = 0;
     __syncthreads();
     const double thr = (double)m - tau;
     uint32_t mil = 0xffffffffu;  // lowest index holding m (given maximum)
-    for (uint32_t i0 = tid; i0 < area; i0 += NT * U) {
-        float v[U];
+    auto take = [&](uint32_t idx, float v) {
+        if (idx < area && (double)v >= thr) {
+            const uint32_t c = atomicAdd(&ncand, 1u);
+            if (c < kRefineCap) cand[c] = idx;
+        }
+        if (smax && idx < area && v == m && idx < mil) mil = idx;
+    };
+    if (smax && cross.y <= (uint32_t)kRowList) {
+        for (uint32_t r = tid; r < cross.y; r += NT) {
+            const uint32_t code = smax[(size_t)sn * cross.y + r];
+            if (code && ((double)screen_max_value(code) >= thr || code == mcode)) rlist[atomicAdd(&nrow, 1u)] = r;
+        }
+        __syncthreads();
+        const uint32_t n = nrow * cross.x;
+        for (uint32_t i0 = tid; i0 < n; i0 += NT * U) {
+            uint32_t idx[U];
+            float v[U];
 #pragma unroll
-        for (int k = 0; k < U; ++k) v[k] = i0 + k * NT < area ? sc[i0 + k * NT] : -__builtin_huge_valf();
-#pragma unroll
-        for (int k = 0; k < U; ++k) {
-            const uint32_t idx = i0 + k * NT;
-            if ((double)v[k] >= thr) {
-                const uint32_t c = atomicAdd(&ncand, 1u);
-                if (c < kRefineCap) cand[c] = idx;
+            for (int k = 0; k < U; ++k) {
+                const uint32_t e = i0 + k * NT;
+                idx[k] = e < n ? (uint32_t)rlist[e / cross.x] * cross.x + e % cross.x : area;
+                v[k] = e < n ? sc[idx[k]] : -__builtin_huge_valf();
             }
-            if (smax && idx < area && v[k] == m && idx < mil) mil = idx;
+#pragma unroll
+            for (int k = 0; k < U; ++k) take(idx[k], v[k]);
+        }
+    } else {
+        for (uint32_t i0 = tid; i0 < area; i0 += NT * U) {
+            float v[U];
+#pragma unroll
+            for (int k = 0; k < U; ++k) v[k] = i0 + k * NT < area ? sc[i0 + k * NT] : -__builtin_huge_valf();
+#pragma unroll
+            for (int k = 0; k < U; ++k) take(i0 + k * NT, v[k]);
         }
     }
     if (smax) mi = block_min_u32<NT>(mil, si);

@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void sum_prepare_kernel(const uint16_t* __rest
                                                           const T* __restrict__ hx, const T* __restrict__ hy,
                                                           HT* __restrict__ hi, HT* __restrict__ lo,
                                                           const float* __restrict__ hxf, const float* __restrict__ hyf,
-                                                          float in_scale, uint32_t* __restrict__ zero,
+                                                          float in_scale, uint32_t* __restrict__ zero, uint32_t zrow,
                                                           const size2* __restrict__ begins, size2 src, size2 slice,
                                                           uint32_t nb, uint32_t total) {
 #pragma clang fp contract(off)
@@ -118,10 +118,9 @@ __global__ __launch_bounds__(256) void sum_prepare_kernel(const uint16_t* __rest
     __syncthreads();
     uint32_t sum = 0;
     for (uint32_t k = 0; k < NT / kWave; ++k) sum += part[k];
-    if (tid == 0) {
-        if (sums) sums[sn] = sum;
-        if (zero) zero[sn] = 0u;
-    }
+    if (tid == 0 && sums) sums[sn] = sum;
+    if (zero)
+        for (uint32_t r = tid; r < zrow; r += NT) zero[(size_t)sn * zrow + r] = 0u;
     // rows of a multiple of 4 pixels: 4 consecutive pixels of one row per thread and
     // step (one 8-byte LDS read, vector table loads and stores); others one pixel
     const bool quad = sx % 4 == 0;
@@ -196,13 +195,13 @@ __global__ __launch_bounds__(256) void sum_prepare_kernel(const uint16_t* __rest
 
 template <class T, class HT>
 bool launch_sum_prepare(const uint16_t* pics, uint32_t* sums, T* slices, const T* hx, const T* hy, HT* hi, HT* lo,
-                        const float* hxf, const float* hyf, float in_scale, uint32_t* zero, const size2* begins,
-                        size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st) {
+                        const float* hxf, const float* hyf, float in_scale, uint32_t* zero, uint32_t zrow,
+                        const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st) {
     const size_t lds = (size_t)slice.area() * sizeof(uint16_t);
     if (lds > kSumPrepareMaxLds) return false;
     if (total == 0) return true;
     sum_prepare_kernel<T, HT><<<total, 256, lds, st>>>(pics, sums, slices, hx, hy, hi, lo, hxf, hyf, in_scale, zero,
-                                                       begins, src, slice, nb, total);
+                                                       zrow, begins, src, slice, nb, total);
     EMIDA_LAUNCHED("sum_prepare_kernel");
     return true;
 }
@@ -228,8 +227,8 @@ void launch_prepare(const IN* pics, T* slices, const T* hx, const T* hy, const S
 
 #define INST_SUM_PREP(T, HT)                                                                              \
     template bool launch_sum_prepare<T, HT>(const uint16_t*, uint32_t*, T*, const T*, const T*, HT*, HT*,   \
-                                            const float*, const float*, float, uint32_t*, const size2*, size2, \
-                                            size2, uint32_t, uint32_t, hipStream_t);
+                                            const float*, const float*, float, uint32_t*, uint32_t,            \
+                                            const size2*, size2, size2, uint32_t, uint32_t, hipStream_t);
 INST_SUM_PREP(double, _Float16)
 INST_SUM_PREP(double, __bf16)
 INST_SUM_PREP(float, _Float16)

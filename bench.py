@@ -545,6 +545,9 @@ def main():
     ap.add_argument("--copy-calibration-mb", type=float, default=0, help="only run a device copy of this size")
     ap.add_argument("--mfma", choices=sorted(MFMA_MODES), default=None,
                     help="brute workloads: matrix-core correlation in this mode instead of the plain form")
+    ap.add_argument("--inflight", type=int, default=1,
+                    help="batches in flight for the batch workloads: one offset object (own stream and "
+                         "scratch) per batch, used in turn")
     ap.add_argument("--no-refine", action="store_true",
                     help="with --mfma: use the float screen values directly (T = float) instead of the exact peak")
     ap.add_argument("--duration", type=float, default=None, help="serving: seconds of offered load")
@@ -615,9 +618,21 @@ def main():
     d_offs = emida_amd.DeviceBuffer(total * 16)
     d_coefs = emida_amd.DeviceBuffer(total * 48)
 
+    # further batches in flight: their own offset objects (streams, scratch, results)
+    gs, outs = [g], [(d_maxes, d_neigh, d_offs, d_coefs)]
+    for _ in range(1, max(1, args.inflight)):
+        g2 = emida_amd.GpuOffset(T, np.uint16, ps.size, begins, slice_size, cross, batch, policy, cfg["fit"],
+                                 mfma_mode=mfma_code, mfma_refine=not args.no_refine)
+        g2.allocate_memory(ps.ref)
+        gs.append(g2)
+        outs.append((emida_amd.DeviceBuffer(total * g.data_index.itemsize),
+                     emida_amd.DeviceBuffer(g.neigh_size * np.dtype(T).itemsize),
+                     emida_amd.DeviceBuffer(total * 16), emida_amd.DeviceBuffer(total * 48)))
+
     def step(i):
         ptr = d_pics.ptr.value + (i % n_batches) * batch * pat_bytes
-        g.process_device(ptr, batch, d_maxes, d_neigh, d_offs, d_coefs)
+        k = i % len(gs)
+        gs[k].process_device(ptr, batch, *outs[k])
 
     for i in range(args.warmup):
         step(i)
@@ -681,7 +696,7 @@ def main():
                       "patterns": n_pat, "rois": n_roi, "roi": f"{2 * r}x{2 * r}",
                       "policy": cfg["policy"] + (f"+mfma:{args.mfma}" + ("" if args.no_refine else "+refine")
                                                  if args.mfma else ""),
-                      "fitsize": cfg["fit"], "batch": batch,
+                      "fitsize": cfg["fit"], "batch": batch, "batches_in_flight": len(gs),
                       "distinct_patterns": args.unique, "device": dev_name},
            "roi_offsets_per_s": round(pats_per_s * n_roi, 1),
            "p50_pattern_latency_ms": round(float(np.percentile(lat, 50)), 4),

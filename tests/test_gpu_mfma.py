@@ -180,6 +180,18 @@ def test_refined_equals_brute_policy_edge_peaks(cross):
         np.testing.assert_array_equal(g, e)
 
 
+@pytest.mark.parametrize("mode", [emida_amd.MFMA_F16_SPLIT, emida_amd.MFMA_BF16])
+def test_refined_equals_brute_policy_tall_window(mode):
+    # 64 x 144 slices with the full 127 x 287 window: more screen rows than the
+    # refinement's row list holds, so every row of the screen is read for candidates
+    ps = synth.PatternSet("mfma-tall", 256, 2, t_range=2.0)
+    begins = [(16, 8), (150, 100), (96, 40)]
+    got = _run(ps, begins, (64, 144), (127, 287), 5, 2, T=np.float64, mode=mode)
+    exp = _run(ps, begins, (64, 144), (127, 287), 5, 2, T=np.float64, policy=emida_amd.CROSS_POLICY_BRUTE)
+    for g, e, what in zip(got, exp, ("offsets", "coefs", "maxes", "neighbourhoods")):
+        np.testing.assert_array_equal(g, e, err_msg=what)
+
+
 def _spread_patterns(batch):
     """384 x 128 uint16 patterns of three 128 x 128 ROIs: vertical stripes (a ridge of
     candidates along the y-shifts), independent noise in pattern and reference (a weak

@@ -199,6 +199,11 @@ bool launch_sum_prepare(const uint16_t* pics, uint32_t* sums, T* slices, const T
                         const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st) {
     const size_t lds = (size_t)slice.area() * sizeof(uint16_t);
     if (lds > kSumPrepareMaxLds) return false;
+    // the four-pixel steps read window tables and write outputs as vectors
+    auto aligned = [](const void* q, size_t a) { return q == nullptr || (uintptr_t)q % a == 0; };
+    if (!aligned(slices, 4 * sizeof(T)) || !aligned(hx, 4 * sizeof(T)) || !aligned(hi, 4 * sizeof(HT)) ||
+        !aligned(lo, 4 * sizeof(HT)) || !aligned(hxf, 4 * sizeof(float)))
+        return false;
     if (total == 0) return true;
     sum_prepare_kernel<T, HT><<<total, 256, lds, st>>>(pics, sums, slices, hx, hy, hi, lo, hxf, hyf, in_scale, zero,
                                                        zrow, begins, src, slice, nb, total);

@@ -296,9 +296,14 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
     }
     // per-row maxima of the values written below (NaN never wins a strict compare): for
     // each y-shift, per accumulator row j the lane's largest value over its two tiles,
-    // then over the 32 lanes of the half (xor < 32 stays in the half); lane j < 16 of
-    // each half publishes pattern row 8 (j/4) + 4 (l/32) + j%4 with one atomic max
+    // then over the 32 lanes of the half by a reduce-scatter (xor 16, 8, 4, 2 halve the
+    // rows a lane keeps, xor 1 pairs the last one: 16 lane exchanges instead of 80), so
+    // lane l ends with row j = 8 b4 + 4 b3 + 2 b2 + b1 (bits of l) and the even lane of
+    // each pair publishes pattern row 8 (j/4) + 4 (l/32) + j%4 with one atomic max
     if (smax) {
+        auto keep_max = [](float& k, float o) {
+            if (o > k) k = o;
+        };
 #pragma unroll
         for (int t = 0; t < TS; ++t) {
             const int row = syw + t + ry;
@@ -316,19 +321,21 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
                 }
             }
 #pragma unroll
-            for (int j = 0; j < 16; ++j)
+            for (int w = 8; w >= 1; w >>= 1) {  // xor 2w: keep the upper w rows if that lane bit is set
+                const bool up = lane & (2 * w);
 #pragma unroll
-                for (int off = 16; off > 0; off >>= 1) {
-                    const float o = __shfl_xor(mx[j], off);
-                    if (o > mx[j]) mx[j] = o;
+                for (int i = 0; i < w; ++i) {
+                    const float lo = mx[i], hi = mx[w + i];
+                    float k = up ? hi : lo;
+                    keep_max(k, __shfl_xor(up ? lo : hi, 2 * w));
+                    mx[i] = k;
                 }
-            float mine = -__builtin_huge_valf();
-#pragma unroll
-            for (int j = 0; j < 16; ++j)
-                if (j == m_a) mine = mx[j];
-            if (m_a < 16 && mine > -__builtin_huge_valf()) {
-                const uint32_t p = p0 + 8 * (m_a / 4) + 4 * half + (m_a % 4);
-                if (p < npat) atomicMax(&smax[((size_t)p * nb + roi) * cross.y + row], screen_max_code(mine));
+            }
+            keep_max(mx[0], __shfl_xor(mx[0], 1));
+            const int j = 8 * ((lane >> 4) & 1) + 4 * ((lane >> 3) & 1) + 2 * ((lane >> 2) & 1) + ((lane >> 1) & 1);
+            if (!(lane & 1) && mx[0] > -__builtin_huge_valf()) {
+                const uint32_t p = p0 + 8 * (j / 4) + 4 * half + (j % 4);
+                if (p < npat) atomicMax(&smax[((size_t)p * nb + roi) * cross.y + row], screen_max_code(mx[0]));
             }
         }
     }

@@ -41,8 +41,11 @@ void lstsq_host(int s, double* out);  // exact (A^T A)^-1 A^T, 6 x s^2
 // with the reference's clamp and position policy, GPU least-squares fit and the
 // finalize formula.  Any output pointer may be null.
 template <class T>
+// row_max (optional, float screens): per-row maximum codes of corr ([slice][cross.y],
+// screen_max_code); the arg-max then reads only the first row holding the maximum.
 void launch_peak(const T* corr, size_t corr_len, uint32_t area, size2 cross, int policy, int fit,
-                 data_index<T>* maxes, T* neigh, double* offsets, double* coefs, uint32_t total, hipStream_t st);
+                 data_index<T>* maxes, T* neigh, double* offsets, double* coefs, uint32_t total, hipStream_t st,
+                 const uint32_t* row_max = nullptr);
 template <class T>
 void launch_maxarg(const T* data, uint32_t area, data_index<T>* out, uint32_t total, hipStream_t st);
 template <class T>

@@ -160,7 +160,7 @@ struct emida_offset_s {
     void* d_hyf = nullptr;
     void* d_pic_hi = nullptr;  // matrix-core policy: prepared pattern slices, hi / lo halves
     void* d_pic_lo = nullptr;
-    void* d_smax = nullptr;    // matrix-core policy with refinement: screen maximum per slice row
+    void* d_smax = nullptr;    // matrix-core policy: screen maximum per slice row
 
     struct outset {
         void* maxes = nullptr;
@@ -214,7 +214,7 @@ struct emida_offset_s {
             parts[2] = al256(tn * cross.area() * sizeof(float));
             parts[3] = al256(tn * slice.area() * 2);
             if (mfma_split(mfma_mode)) parts[4] = parts[3];
-            if (mfma_refine) parts[5] = al256(tn * cross.y * sizeof(uint32_t));
+            parts[5] = al256(tn * cross.y * sizeof(uint32_t));
         } else if (fft_padded(side) > 128) {
             parts[2] = al256(fft256_work_bytes(nb, (uint32_t)tn));  // column-stage work buffer
         }
@@ -409,7 +409,7 @@ struct emida_offset_s {
                 d_pic_hi = alloc(scratch_total * slice.area() * half);
                 if (mfma_split(mfma_mode)) d_pic_lo = alloc(scratch_total * slice.area() * half);
                 d_corr = alloc(scratch_total * cross.area() * sizeof(float));
-                if (mfma_refine) d_smax = alloc(scratch_total * cross.y * sizeof(uint32_t));
+                d_smax = alloc(scratch_total * cross.y * sizeof(uint32_t));
             }
         } else {
             const size_t m = fft_padded(side);
@@ -505,7 +505,7 @@ struct emida_offset_s {
                     } else {
                         launch_peak<float>((const float*)sc.corr, (size_t)tn * cross.area(), cross.area(), cross,
                                            POLICY_BRUTE, fit, (data_index<float>*)maxes, (float*)neigh, offsets,
-                                           coefs, tn, st);
+                                           coefs, tn, st, (const uint32_t*)sc.smax);
                     }
                 } else {
                     if (!std::is_same<IT, uint16_t>::value)

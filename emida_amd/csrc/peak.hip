@@ -149,7 +149,7 @@ __global__ __launch_bounds__(NT) void peak_kernel(const T* __restrict__ corr, si
                                                   size2 cross, int policy, int fit, const double* __restrict__ lsq,
                                                   data_index<T>* __restrict__ maxes, T* __restrict__ neigh,
                                                   double* __restrict__ offsets, double* __restrict__ coefs,
-                                                  uint32_t total) {
+                                                  uint32_t total, const uint32_t* __restrict__ row_max) {
     __shared__ T sv[NT / kWave];
     __shared__ uint32_t si[NT / kWave];
     __shared__ T nbv[81];
@@ -159,14 +159,38 @@ __global__ __launch_bounds__(NT) void peak_kernel(const T* __restrict__ corr, si
     const T* d = corr + (size_t)sn * area;
     T best = lowest<T>();
     uint32_t bi = 0xffffffffu;
-    for (uint32_t i = threadIdx.x; i < area; i += NT) {
-        const T v = d[i];
-        if (v > best) {
-            best = v;
-            bi = i;
+    if (row_max) {
+        // the maximum from the row maxima; its lowest index lies in the first row whose
+        // maximum equals it, so only that row is read
+        const uint32_t* rm = row_max + (size_t)sn * cross.y;
+        uint32_t mc = 0;
+        for (uint32_t r = threadIdx.x; r < cross.y; r += NT) mc = max(mc, rm[r]);
+        mc = ~block_min_u32<NT>(~mc, si);
+        const T m = mc ? (T)screen_max_value(mc) : lowest<T>();
+        if (m > lowest<T>()) {
+            uint32_t row = 0xffffffffu, x = 0xffffffffu;
+            for (uint32_t r = threadIdx.x; r < cross.y; r += NT)
+                if (rm[r] == mc) row = min(row, r);
+            row = block_min_u32<NT>(row, si);
+            for (uint32_t i = threadIdx.x; i < cross.x; i += NT)
+                if (d[(size_t)row * cross.x + i] == m) x = min(x, i);
+            x = block_min_u32<NT>(x, si);
+            if (x != 0xffffffffu) {
+                best = m;
+                bi = row * cross.x + x;
+            }
         }
     }
-    block_argmax<T, NT>(best, bi, sv, si);
+    if (bi == 0xffffffffu) {
+        for (uint32_t i = threadIdx.x; i < area; i += NT) {
+            const T v = d[i];
+            if (v > best) {
+                best = v;
+                bi = i;
+            }
+        }
+        block_argmax<T, NT>(best, bi, sv, si);
+    }
     const int k = fit * fit;
     const size2 mp = shift_pos(policy, bi, cross);
     const int r = (fit - 1) / 2;
@@ -636,11 +660,12 @@ constexpr int kPeakNT = 256;
 
 template <class T>
 void launch_peak(const T* corr, size_t corr_len, uint32_t area, size2 cross, int policy, int fit,
-                 data_index<T>* maxes, T* neigh, double* offsets, double* coefs, uint32_t total, hipStream_t st) {
+                 data_index<T>* maxes, T* neigh, double* offsets, double* coefs, uint32_t total, hipStream_t st,
+                 const uint32_t* row_max) {
     const double* lsq = lstsq_device_table() + lstsq_offset(fit);
     if (total == 0) return;
     peak_kernel<T, kPeakNT><<<total, kPeakNT, 0, st>>>(corr, corr_len, area, cross, policy, fit, lsq, maxes, neigh,
-                                                      offsets, coefs, total);
+                                                      offsets, coefs, total, row_max);
     EMIDA_LAUNCHED("peak_kernel");
 }
 
@@ -692,7 +717,7 @@ template void launch_refine<double>(const float*, size_t, const double*, const d
 
 #define INST_PEAK(T)                                                                                          \
     template void launch_peak<T>(const T*, size_t, uint32_t, size2, int, int, data_index<T>*, T*, double*,     \
-                                 double*, uint32_t, hipStream_t);                                             \
+                                 double*, uint32_t, hipStream_t, const uint32_t*);                            \
     template void launch_maxarg<T>(const T*, uint32_t, data_index<T>*, uint32_t, hipStream_t);                \
     template void launch_extract<T>(const T*, size_t, const data_index<T>*, T*, int, size2, int, uint32_t,    \
                                     hipStream_t);                                                             \

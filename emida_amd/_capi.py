@@ -107,6 +107,11 @@ def lib():
             "emida_offset_finalize": ([vp, vp, vp, dp, dp], i32),
             "emida_offset_get_offset": ([vp, vp, dp, dp], i32),
             "emida_offset_process_device": ([vp, vp, u32, vp, vp, vp, vp, vp], i32),
+            "emida_offset_get_cu_pic": ([vp, ctypes.POINTER(vp)], i32),
+            "emida_offset_get_cu_temp": ([vp, ctypes.POINTER(vp)], i32),
+            "emida_offset_cross_corr_fft": ([vp], i32),
+            "emida_offset_get_cu_cross_res": ([vp, ctypes.POINTER(vp)], i32),
+            "emida_offset_get_streams": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp)], i32),
             "emida_offset_set_mfma_mode": ([vp, i32], i32),
             "emida_offset_set_mfma_refine": ([vp, i32], i32),
             "emida_offset_set_timing": ([vp, i32], i32),
@@ -121,6 +126,7 @@ def lib():
             "emida_memset_device": ([vp, i32, sz], i32),
             "emida_device_synchronize": ([], i32),
             "emida_device_query": ([ctypes.c_char_p, sz, ctypes.POINTER(i32)], i32),
+            "emida_device_mem_info": ([ctypes.POINTER(sz), ctypes.POINTER(sz)], i32),
             "emida_run_sum": ([i32, vp, vp, vp, S, S, u32, u32, vp], i32),
             "emida_run_prepare_pics": ([i32, i32, i32, vp, vp, vp, vp, vp, vp, S, S, S, u32, u32, vp], i32),
             "emida_run_cross_corr": ([i32, vp, vp, vp, S, S, u32, u32, vp], i32),

@@ -42,7 +42,12 @@ int emida_offset_create(emida_offset** out, emida_type T, emida_type IN, emida_s
         h->begins.resize(n_begins);
         for (uint32_t i = 0; i < n_begins; ++i) h->begins[i] = S2(begins[i]);
         h->sum_bytes = IN == EMIDA_U16 ? 4 : type_size(IN);
-        if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA) check_mfma_shape(S2(slice_size), S2(cross_size));
+        if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA) {
+            check_mfma_shape(S2(slice_size), S2(cross_size));
+            // fp16 operands hold |prepared value| < 131008 only, which uint16 patterns
+            // guarantee; float and double patterns default to bf16 (float's range)
+            if (IN != EMIDA_U16) h->mfma_mode = EMIDA_MFMA_BF16_SPLIT;
+        }
         if (policy != EMIDA_CROSS_POLICY_FFT) {
             if (cross_size.x == 0 || cross_size.y == 0) throw arg_error("cross size must be positive");
             if (cross_size.x > 2 * slice_size.x - 1 || cross_size.y > 2 * slice_size.y - 1)
@@ -50,18 +55,16 @@ int emida_offset_create(emida_offset** out, emida_type T, emida_type IN, emida_s
             h->cross = S2(cross_size);
             h->buf = h->cross;
         } else {
-            // the FFT policy forces cross_size = 2s - 1 (src/gpu_offset.hpp:114)
-            if (slice_size.x != slice_size.y)
-                throw unsupported_error("FFT policy: square slices only");
-            // any side up to half the largest transform; non-powers of two are
-            // zero-padded to the next power of two (same correlation values)
-            const uint32_t s = slice_size.x;
-            if (s < 2 || s > (uint32_t)fft_max_side(T == EMIDA_F64) / 2)
-                throw unsupported_error("FFT policy: slice side must be between 2 and " +
-                                        std::to_string(fft_max_side(T == EMIDA_F64) / 2));
-            h->side = s;
-            h->cross = {2 * s - 1, 2 * s - 1};
-            h->buf = {2 * s, 2 * s};
+            // the FFT policy forces cross_size = 2s - 1 (src/gpu_offset.hpp:114); any
+            // side per axis up to kFftMaxSlice, each zero-padded to the next power of
+            // two >= 2s (same correlation values)
+            if (slice_size.x > kFftMaxSlice || slice_size.y > kFftMaxSlice)
+                throw unsupported_error("FFT policy: slice sides must be between 2 and " +
+                                        std::to_string(kFftMaxSlice));
+            h->side = slice_size.x;
+            h->fftg = fftg_needed(T == EMIDA_F64, S2(slice_size));
+            h->cross = {2 * slice_size.x - 1, 2 * slice_size.y - 1};
+            h->buf = {2 * slice_size.x, 2 * slice_size.y};
         }
         *out = h.release();
     });
@@ -150,11 +153,53 @@ int emida_offset_process_device(emida_offset* h, const void* d_pics, uint32_t n_
     });
 }
 
+int emida_offset_get_cu_pic(emida_offset* h, void** dptr) {
+    return guarded([&] {
+        if (!h || !dptr) throw arg_error("null argument");
+        *dptr = h->cu_pic();
+    });
+}
+
+int emida_offset_get_cu_temp(emida_offset* h, void** dptr) {
+    return guarded([&] {
+        if (!h || !dptr) throw arg_error("null argument");
+        *dptr = h->cu_temp();
+    });
+}
+
+int emida_offset_cross_corr_fft(emida_offset* h) {
+    return guarded([&] {
+        if (!h) throw arg_error("null object");
+        h->cross_corr_fft();
+    });
+}
+
+int emida_offset_get_cu_cross_res(emida_offset* h, void** dptr) {
+    return guarded([&] {
+        if (!h || !dptr) throw arg_error("null argument");
+        h->seam_init();
+        *dptr = h->d_seam_res;
+    });
+}
+
+int emida_offset_get_streams(emida_offset* h, void** in_stream, void** out_stream) {
+    return guarded([&] {
+        if (!h) throw arg_error("null object");
+        h->require_allocated();
+        if (in_stream) *in_stream = (void*)h->in_stream;
+        if (out_stream) *out_stream = (void*)h->out_stream;
+    });
+}
+
 int emida_offset_set_mfma_mode(emida_offset* h, emida_mfma_mode mode) {
     return guarded([&] {
         if (!h) throw arg_error("null object");
         if (h->allocated) throw state_error("set_mfma_mode must precede allocate_memory");
         with_HT(mode, [](auto) {});
+        if ((mode == EMIDA_MFMA_F16 || mode == EMIDA_MFMA_F16_SPLIT) && h->IN != EMIDA_U16)
+            throw unsupported_error(
+                "fp16 matrix-core operands hold prepared values below 131008 in magnitude, which only uint16 "
+                "patterns guarantee; use a bf16 mode for float or double patterns");
         h->mfma_mode = mode;
     });
 }
@@ -191,7 +236,9 @@ int emida_offset_get_timing(emida_offset* h, const char** kernel_name, double* t
         h->tev.clear();
         if (kernel_name)
             *kernel_name = h->policy == EMIDA_CROSS_POLICY_FFT
-                               ? (fft_padded(h->side) > 128 ? "fft256_cols_kernel+fft256_rows_kernel" : "fft_kernel")
+                               ? (h->fftg ? "fftg_rows_kernel+fftg_cols_kernel+fftg_rows_inv_kernel+peak_kernel"
+                                          : fft_padded(h->side) > 128 ? "fft256_cols_kernel+fft256_rows_kernel"
+                                                                      : "fft_kernel")
                                : (h->policy == EMIDA_CROSS_POLICY_BRUTE_MFMA
                                       ? "mfma_corr_kernel"
                                       : cross_corr_kernel_name(h->T == EMIDA_F64, h->slice, h->cross, h->nb, h->batch));
@@ -244,6 +291,15 @@ int emida_device_query(char* name, size_t name_len, int* cu_count) {
             std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
         }
         if (cu_count) *cu_count = prop.multiProcessorCount;
+    });
+}
+
+int emida_device_mem_info(size_t* free_bytes, size_t* total_bytes) {
+    return guarded([&] {
+        size_t f = 0, t = 0;
+        EMIDA_HIP(hipMemGetInfo(&f, &t));
+        if (free_bytes) *free_bytes = f;
+        if (total_bytes) *total_bytes = t;
     });
 }
 
@@ -312,10 +368,9 @@ int emida_run_cross_corr_fft(emida_type T, const void* pics, const void* ref, vo
                              uint32_t ref_slices, uint32_t batch_size, void* stream) {
     return guarded([&] {
         if (!pics || !ref || !res) throw arg_error("null argument");
-        if (size.x != size.y) throw unsupported_error("cross_corr_fft: square slices only");
         with_T(T, [&](auto t) {
             using TT = decltype(t);
-            launch_fft_corr_prepared<TT>((const TT*)pics, (const TT*)ref, (TT*)res, size.x, ref_slices, batch_size,
+            launch_fft_corr_prepared<TT>((const TT*)pics, (const TT*)ref, (TT*)res, S2(size), ref_slices, batch_size,
                                          (hipStream_t)stream);
         });
     });
@@ -404,7 +459,7 @@ int emida_fft_max_slice(emida_type T, uint32_t* side) {
     return guarded([&] {
         if (!side) throw arg_error("null argument");
         if (T != EMIDA_F32 && T != EMIDA_F64) throw arg_error("T must be F32 or F64");
-        *side = (uint32_t)fft_max_side(T == EMIDA_F64) / 2;
+        *side = kFftMaxSlice;
     });
 }
 

@@ -467,13 +467,11 @@ void launch_fft_pipe(const IN* pics, const S* sums, const T* hx, const T* hy, co
     launch_fft_side<T, IN, S, true, true>(side, a, total, st);
 }
 
+// Reference spectra from prepared reference slices, then the correlation of every
+// prepared pattern slice; `work` and `chunk` serve the 256-point path only.
 template <class T>
-void launch_fft_corr_prepared(const T* pics, const T* ref, T* res, uint32_t side, uint32_t ref_slices,
-                              uint32_t batch, hipStream_t st) {
-    if (ref_slices == 0 || batch == 0) return;
-    const size_t m = fft_padded(side), nc = m / 2 + 1;
-    cplx<T>* spectra = nullptr;
-    EMIDA_HIP(hipMallocAsync((void**)&spectra, nc * m * ref_slices * sizeof(cplx<T>), st));
+static void fft_corr_prepared(const T* pics, const T* ref, T* res, uint32_t side, uint32_t ref_slices,
+                              uint32_t batch, cplx<T>* spectra, void* work, uint32_t chunk, hipStream_t st) {
     fft_args<T, T, T> a{};
     a.nb = ref_slices;
     a.total = ref_slices;
@@ -488,11 +486,45 @@ void launch_fft_corr_prepared(const T* pics, const T* ref, T* res, uint32_t side
     b.fit = 3;
     b.lsq = lstsq_device_table();
     b.corr_out = res;
-    if (m > 128) EMIDA_HIP(hipMallocAsync((void**)&b.work, fft256_work_bytes(b.nb, b.total), st));
+    b.work = (cplx<T>*)work;
+    b.chunk = chunk;
     launch_fft_side<T, T, T, false, true>(side, b, b.total, st);
-    if (b.work) EMIDA_HIP(hipFreeAsync(b.work, st));
+}
+
+template <class T>
+static size_t spectra_bytes(uint32_t side, uint32_t ref_slices) {
+    const size_t m = fft_padded(side);
+    return (m / 2 + 1) * m * ref_slices * sizeof(cplx<T>);
+}
+
+template <class T>
+void launch_fft_corr_prepared(const T* pics, const T* ref, T* res, size2 side, uint32_t ref_slices, uint32_t batch,
+                              hipStream_t st) {
+    if (ref_slices == 0 || batch == 0) return;
+    if (side.x < 2 || side.y < 2 || side.x > kFftMaxSlice || side.y > kFftMaxSlice)
+        throw unsupported_error("cross_corr_fft: slice sides must be between 2 and " + std::to_string(kFftMaxSlice));
+    const bool dbl = sizeof(T) == 8;
+    cplx<T>* spectra = nullptr;
+    void* work = nullptr;
+    if (fftg_needed(dbl, side)) {
+        const size_t chunk = fftg_chunk(dbl, side, std::max(ref_slices, ref_slices * batch), 0);
+        EMIDA_HIP(hipMallocAsync((void**)&spectra, fftg_spectrum_bytes(dbl, side) * ref_slices, st));
+        EMIDA_HIP(hipMallocAsync(&work, chunk * fftg_slice_work_bytes(dbl, side), st));
+        launch_fftg_corr_prepared<T>(pics, ref, res, side, ref_slices, batch, spectra, work, chunk, st);
+    } else {
+        EMIDA_HIP(hipMallocAsync((void**)&spectra, spectra_bytes<T>(side.x, ref_slices), st));
+        if (fft_padded(side.x) > 128)
+            EMIDA_HIP(hipMallocAsync(&work, fft256_work_bytes(ref_slices, ref_slices * batch), st));
+        fft_corr_prepared<T>(pics, ref, res, side.x, ref_slices, batch, spectra, work, 0, st);
+    }
+    if (work) EMIDA_HIP(hipFreeAsync(work, st));
     EMIDA_HIP(hipFreeAsync(spectra, st));
 }
+
+#define INST_CORR_PREPARED(T)                                                                                   \
+    template void launch_fft_corr_prepared<T>(const T*, const T*, T*, size2, uint32_t, uint32_t, hipStream_t);
+INST_CORR_PREPARED(float)
+INST_CORR_PREPARED(double)
 
 // p = p * conj(r), the reference formula (src/hadamard.cu:39-40); ref repeats
 // every one.area() * ref_slices elements.
@@ -526,10 +558,6 @@ INST_FFT(double, double, double)
 INST_FFT(float, double, double)
 INST_FFT(double, float, float)
 
-template void launch_fft_corr_prepared<float>(const float*, const float*, float*, uint32_t, uint32_t, uint32_t,
-                                              hipStream_t);
-template void launch_fft_corr_prepared<double>(const double*, const double*, double*, uint32_t, uint32_t,
-                                               uint32_t, hipStream_t);
 template void launch_hadamard<float>(cplx<float>*, const cplx<float>*, size2, uint32_t, uint32_t, hipStream_t);
 template void launch_hadamard<double>(cplx<double>*, const cplx<double>*, size2, uint32_t, uint32_t,
                                       hipStream_t);

@@ -372,6 +372,8 @@ size_t fft256_chunk(uint32_t nb) {
     return (per < 1 ? 1 : per) * (size_t)nb;
 }
 
+size_t fft256_slice_work_bytes() { return (size_t)kNC * kM * sizeof(cplx<float>); }
+
 size_t fft256_work_bytes(uint32_t nb, uint32_t total) {
     const size_t chunk = fft256_chunk(nb);
     return (total < chunk ? total : chunk) * (size_t)kNC * kM * sizeof(cplx<float>);
@@ -393,7 +395,8 @@ void launch_fft256(fft_args<float, IN, SUM> a, hipStream_t st) {
         static const hipError_t attr2 =
             hipFuncSetAttribute((const void*)k2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLds2);
         EMIDA_HIP(attr2);
-        const uint32_t total = a.total, chunk = (uint32_t)fft256_chunk(a.nb);
+        const uint32_t total = a.total, chunk = a.chunk ? a.chunk : (uint32_t)fft256_chunk(a.nb);
+        if (chunk % a.nb != 0) throw arg_error("FFT 256: launch chunk must be a multiple of the ROI count");
         for (uint32_t s0 = 0; s0 < total; s0 += chunk) {
             fft_args<float, IN, SUM> b = a;
             b.first = s0;

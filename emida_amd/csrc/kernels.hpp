@@ -72,15 +72,18 @@ void launch_refine(const float* screen, size_t corr_len, const T* p, const T* r,
 void mfma_refine_bounds(int mode, size2 s, bool t_double, double& tol_rel, double& tol_abs);
 
 // ---- fft.hip, fft256.hip
-// Largest padded FFT side: 128 for double (single-kernel pipeline), 256 for float
-// (M = 256 runs as two kernels with a work buffer, see fft256.hip).
+// Largest padded FFT side of the single-kernel pipelines: 128 for double, 256 for
+// float (M = 256 runs as two kernels with a work buffer, see fft256.hip); fftg.hip
+// takes every other shape up to kFftMaxSlice per axis.
 int fft_max_side(bool is_double);
+constexpr uint32_t kFftMaxSlice = 256;
 // Transform side M of a slice side s: the power of two >= 2s (at least 4).
 uint32_t fft_padded(uint32_t side);
 // M = 256: slices per launch chunk and the work buffer a launch of `total` slices needs.
 constexpr uint32_t kFft256Chunk = 4096;
 size_t fft256_chunk(uint32_t nb);
 size_t fft256_work_bytes(uint32_t nb, uint32_t total);
+size_t fft256_slice_work_bytes();  // work buffer bytes per slice
 // Reference spectra: prepared reference slices -> half spectra in the layout the
 // pipeline kernel reads ((M/2+1) columns x M bit-reversed rows per ROI).
 template <class T, class IN, class S>
@@ -97,11 +100,42 @@ void launch_fft_pipe(const IN* pics, const S* sums, const T* hx, const T* hy, co
                      hipStream_t st);
 // Prepared slices in, circular correlation buffers out (cross_corr_fft seam).
 template <class T>
-void launch_fft_corr_prepared(const T* pics, const T* ref, T* res, uint32_t side, uint32_t ref_slices,
-                              uint32_t batch, hipStream_t st);
+void launch_fft_corr_prepared(const T* pics, const T* ref, T* res, size2 side, uint32_t ref_slices, uint32_t batch,
+                              hipStream_t st);
 template <class T>
 void launch_hadamard(cplx<T>* pics, const cplx<T>* ref, size2 one, uint32_t ref_slices, uint32_t batch,
                      hipStream_t st);
+
+// ---- fftg.hip: three-kernel FFT path (non-square slices, double sides above 64, sides
+// above 128; transforms up to 512 per axis), work buffers of half spectra in HBM
+bool fftg_needed(bool is_double, size2 side);               // the single-kernel paths do not take this shape
+size_t fftg_spectrum_bytes(bool is_double, size2 side);      // reference spectrum of one ROI
+size_t fftg_slice_work_bytes(bool is_double, size2 side);    // half spectra of one slice
+size_t fftg_chunk(bool is_double, size2 side, uint32_t total, size_t extra_per_slice);  // slices per launch chunk
+size_t fftg_ref_work_bytes(bool is_double, size2 side, uint32_t nb);  // work of launch_fftg_ref
+// work of launch_fftg_pipe (with_corr: it also holds the chunk's correlation buffers)
+size_t fftg_work_bytes(bool is_double, size2 side, uint32_t total, bool with_corr);
+template <class T, class IN, class S>
+void launch_fftg_ref(const IN* pics, const S* sums, const T* hx, const T* hy, const size2* begins, size2 src,
+                     size2 side, uint32_t nb, cplx<T>* spectra, void* work, hipStream_t st);
+// as launch_fft_pipe; work: fftg_work_bytes(.., total, true) bytes
+template <class T, class IN, class S>
+void launch_fftg_pipe(const IN* pics, const S* sums, const T* hx, const T* hy, const size2* begins, size2 src,
+                      size2 side, uint32_t nb, uint32_t total, const cplx<T>* spectra, int fit, data_index<T>* maxes,
+                      T* neigh, double* offsets, double* coefs, T* corr_out, void* work, hipStream_t st);
+// prepared slices -> (2s.x x 2s.y) circular correlation buffers; spectra: nb ROIs'
+// spectra (fftg_spectrum_bytes each), work: `chunk` slices of half spectra
+template <class T>
+void launch_fftg_corr_prepared(const T* pics, const T* ref, T* res, size2 side, uint32_t nb, uint32_t batch,
+                               cplx<T>* spectra, void* work, size_t chunk, hipStream_t st);
+// cuFFT-layout R2C (spectra [n][2s.y][s.x + 1]) and R2C -> product -> C2R of
+// 2s.x x 2s.y buffers (the reference's cross_corr_fft); 2s a power of two per axis
+template <class T>
+void launch_fftg_std_r2c(const T* pic, uint32_t pitch, cplx<T>* spectra, size2 side, uint32_t n, void* work,
+                         size_t chunk, hipStream_t st);
+template <class T>
+void launch_fftg_std_corr(const T* pic, const cplx<T>* spectra, T* res, size2 side, uint32_t nb, uint32_t total,
+                          void* work, size_t chunk, hipStream_t st);
 
 // ---- mfma.hip (matrix-core brute correlation, float results)
 // Prepared slices as half-type hi (+ lo when lo != nullptr) planes [slice][y][x];

@@ -131,7 +131,8 @@ struct emida_offset_s {
     int fit;
     size2 src, slice, cross, buf;
     uint32_t nb, batch, total, n_in;
-    uint32_t side = 0;  // FFT slice side
+    uint32_t side = 0;  // FFT slice side (x)
+    bool fftg = false;  // FFT policy through the three-kernel path (fftg.hip)
     std::vector<size2> begins;
     size_t sum_bytes = 0;
 
@@ -161,6 +162,14 @@ struct emida_offset_s {
     void* d_pic_hi = nullptr;  // matrix-core policy: prepared pattern slices, hi / lo halves
     void* d_pic_lo = nullptr;
     void* d_smax = nullptr;    // matrix-core policy: screen maximum per slice row
+    // the reference's cross_corr_fft seam (get_cu_pic / get_cu_temp / cross_corr_fft,
+    // src/gpu_offset.hpp:303-325), FFT policy: made on first use
+    void* d_seam_pic = nullptr;   // total x (2s.y rows of 2s.x) T
+    void* d_seam_temp = nullptr;  // nb x (2s.y rows of s.x + 1) complex T: R2C layout
+    void* d_seam_res = nullptr;   // total x (2s.y rows of 2s.x) T
+    void* d_seam_work = nullptr;
+    size_t seam_chunk = 0;
+    std::vector<unsigned char> h_ref;  // FFT policy: the reference pattern, for the seam's spectra
 
     struct outset {
         void* maxes = nullptr;
@@ -215,6 +224,8 @@ struct emida_offset_s {
             parts[3] = al256(tn * slice.area() * 2);
             if (mfma_split(mfma_mode)) parts[4] = parts[3];
             parts[5] = al256(tn * cross.y * sizeof(uint32_t));
+        } else if (fftg) {
+            parts[2] = al256(fftg_work_bytes(T == EMIDA_F64, slice, (uint32_t)tn, true));  // half spectra + corr
         } else if (fft_padded(side) > 128) {
             parts[2] = al256(fft256_work_bytes(nb, (uint32_t)tn));  // column-stage work buffer
         }
@@ -257,8 +268,9 @@ struct emida_offset_s {
         for (void* p : d_in) (void)hipFree(p);
         for (auto e : ev_up) (void)hipEventDestroy(e);
         for (auto e : ev_done) (void)hipEventDestroy(e);
-        void* ptrs[] = {d_begins, d_hx,     d_hy,     d_sums, d_ref,  d_slices, d_corr,
-                        d_ref_lo, d_pic_hi, d_pic_lo, d_ref_t, d_hxf,  d_hyf};
+        void* ptrs[] = {d_begins, d_hx,     d_hy,     d_sums,  d_ref, d_slices, d_corr,
+                        d_ref_lo, d_pic_hi, d_pic_lo, d_ref_t, d_hxf, d_hyf,    d_smax,
+                        d_seam_pic, d_seam_temp, d_seam_res, d_seam_work};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         for (auto& o : out) {
@@ -347,8 +359,22 @@ struct emida_offset_s {
                 } else {
                     if (!std::is_same<IT, uint16_t>::value)
                         launch_sum<IT, ST>(pic, sums, d_begins, src, slice, nb, nb, compute);
-                    launch_fft_ref<TT, IT, ST>(pic, sums, (const TT*)d_hx, (const TT*)d_hy, d_begins, src, side, nb,
-                                               (cplx<TT>*)d_ref, compute);
+                    if (fftg) {
+                        void* work = dmalloc(fftg_ref_work_bytes(T == EMIDA_F64, slice, nb));
+                        try {
+                            launch_fftg_ref<TT, IT, ST>(pic, sums, (const TT*)d_hx, (const TT*)d_hy, d_begins, src,
+                                                        slice, nb, (cplx<TT>*)d_ref, work, compute);
+                            EMIDA_HIP(hipStreamSynchronize(compute));
+                        } catch (...) {
+                            (void)hipStreamSynchronize(compute);
+                            (void)hipFree(work);
+                            throw;
+                        }
+                        EMIDA_HIP(hipFree(work));
+                    } else {
+                        launch_fft_ref<TT, IT, ST>(pic, sums, (const TT*)d_hx, (const TT*)d_hy, d_begins, src, side,
+                                                   nb, (cplx<TT>*)d_ref, compute);
+                    }
                 }
             });
         });
@@ -411,6 +437,9 @@ struct emida_offset_s {
                 d_corr = alloc(scratch_total * cross.area() * sizeof(float));
                 d_smax = alloc(scratch_total * cross.y * sizeof(uint32_t));
             }
+        } else if (fftg) {
+            d_ref = alloc(fftg_spectrum_bytes(T == EMIDA_F64, slice) * nb);
+            if (!resident_only) d_corr = alloc(fftg_work_bytes(T == EMIDA_F64, slice, total, true));
         } else {
             const size_t m = fft_padded(side);
             const size_t spec = (m / 2 + 1) * m * 2 * t_bytes();
@@ -428,7 +457,86 @@ struct emida_offset_s {
         }
         (void)lstsq_device_table();
         if (temp) prepare_reference(temp);
+        if (temp && policy == EMIDA_CROSS_POLICY_FFT && !resident_only)
+            h_ref.assign((const unsigned char*)temp, (const unsigned char*)temp + (size_t)src.area() * in_bytes());
         allocated = true;
+    }
+
+    // ---- the reference's cross_corr_fft seam, FFT policy (src/gpu_offset.hpp:303-325)
+    void seam_init() {
+        require_allocated();
+        if (policy != EMIDA_CROSS_POLICY_FFT) throw state_error("cross_corr_fft belongs to the FFT policy");
+        if (d_seam_pic) return;
+        const bool dbl = T == EMIDA_F64;
+        if (fft_padded(slice.x) != 2 * slice.x || fft_padded(slice.y) != 2 * slice.y)
+            throw unsupported_error("cross_corr_fft, get_cu_pic, get_cu_temp: 2s must be a power of two per axis");
+        const size_t buf = (size_t)4 * slice.area() * t_bytes();
+        const size_t spec = (size_t)(slice.x + 1) * 2 * slice.y * 2 * t_bytes();
+        seam_chunk = fftg_chunk(dbl, slice, std::max(total, nb), 0);
+        d_seam_pic = alloc(buf * total);
+        d_seam_temp = alloc(spec * nb);
+        d_seam_res = alloc(buf * std::max(total, nb));
+        d_seam_work = alloc(seam_chunk * fftg_slice_work_bytes(dbl, slice));
+        EMIDA_HIP(hipMemsetAsync(d_seam_pic, 0, buf * total, compute));
+        EMIDA_HIP(hipMemsetAsync(d_seam_temp, 0, spec * nb, compute));
+        if (!h_ref.empty()) {
+            // R2C of the prepared reference, zero-padded to 2s.x x 2s.y (the reference's
+            // in-place transform of cu_temp_, src/gpu_offset.hpp:185-210)
+            void* d_temp = dmalloc(h_ref.size());
+            try {
+                EMIDA_HIP(hipMemcpyAsync(d_temp, h_ref.data(), h_ref.size(), hipMemcpyHostToDevice, compute));
+                EMIDA_HIP(hipMemsetAsync(d_seam_res, 0, buf * nb, compute));
+                with_T(T, [&](auto t) {
+                    using TT = decltype(t);
+                    with_IN(IN, [&](auto in, auto sm) {
+                        using IT = decltype(in);
+                        using ST = decltype(sm);
+                        launch_sum<IT, ST>((const IT*)d_temp, (ST*)d_sums, d_begins, src, slice, nb, nb, compute);
+                        launch_prepare<IT, TT, ST>((const IT*)d_temp, (TT*)d_seam_res, (const TT*)d_hx,
+                                                   (const TT*)d_hy, (const ST*)d_sums, d_begins, src, slice,
+                                                   size2{2 * slice.x, 2 * slice.y}, nb, nb, compute);
+                        launch_fftg_std_r2c<TT>((const TT*)d_seam_res, 2 * slice.x, (cplx<TT>*)d_seam_temp, slice, nb,
+                                                d_seam_work, seam_chunk, compute);
+                    });
+                });
+                EMIDA_HIP(hipStreamSynchronize(compute));
+            } catch (...) {
+                (void)hipStreamSynchronize(compute);
+                (void)hipFree(d_temp);
+                throw;
+            }
+            EMIDA_HIP(hipFree(d_temp));
+        }
+        EMIDA_HIP(hipStreamSynchronize(compute));
+    }
+
+    void* cu_pic() {
+        require_allocated();
+        if (policy == EMIDA_CROSS_POLICY_FFT) {
+            seam_init();
+            return d_seam_pic;
+        }
+        if (!d_slices) throw state_error("get_cu_pic: this configuration keeps no prepared pattern slices");
+        return d_slices;
+    }
+    void* cu_temp() {
+        require_allocated();
+        if (policy == EMIDA_CROSS_POLICY_FFT) {
+            seam_init();
+            return d_seam_temp;
+        }
+        void* p = policy == EMIDA_CROSS_POLICY_BRUTE ? d_ref : d_ref_t;
+        if (!p) throw state_error("get_cu_temp: this configuration keeps no T-precision reference slices");
+        return p;
+    }
+    void cross_corr_fft() {
+        seam_init();
+        with_T(T, [&](auto t) {
+            using TT = decltype(t);
+            launch_fftg_std_corr<TT>((const TT*)d_seam_pic, (const cplx<TT>*)d_seam_temp, (TT*)d_seam_res, slice, nb,
+                                     total, d_seam_work, seam_chunk, compute);
+        });
+        EMIDA_HIP(hipStreamSynchronize(compute));
     }
 
     // Sums and preparation of a batch in one kernel for uint16 patterns (uint32 sums);
@@ -511,9 +619,16 @@ struct emida_offset_s {
                     if (!std::is_same<IT, uint16_t>::value)
                         launch_sum<IT, ST>(pics, sums, d_begins, src, slice, nb, tn, st);
                     if (timing) t0 = record_start(st);
-                    launch_fft_pipe<TT, IT, ST>(pics, sums, (const TT*)d_hx, (const TT*)d_hy, d_begins, src, side,
-                                                nb, tn, (const cplx<TT>*)d_ref, fit, (data_index<TT>*)maxes,
-                                                (TT*)neigh, offsets, coefs, nullptr, sc.corr, st);
+                    if (fftg)
+                        launch_fftg_pipe<TT, IT, ST>(pics, sums, (const TT*)d_hx, (const TT*)d_hy, d_begins, src,
+                                                     slice, nb, tn, (const cplx<TT>*)d_ref, fit,
+                                                     (data_index<TT>*)maxes, (TT*)neigh, offsets, coefs, nullptr,
+                                                     sc.corr, st);
+                    else
+                        launch_fft_pipe<TT, IT, ST>(pics, sums, (const TT*)d_hx, (const TT*)d_hy, d_begins, src,
+                                                    side, nb, tn, (const cplx<TT>*)d_ref, fit,
+                                                    (data_index<TT>*)maxes, (TT*)neigh, offsets, coefs, nullptr,
+                                                    sc.corr, st);
                     if (timing) record_stop(st, t0);
                 }
             });

@@ -291,11 +291,13 @@ __global__ void subpixel_kernel(const T* __restrict__ neigh, int s, uint32_t n, 
 //   arg-max over the candidates' exact values, lowest index on ties;
 //   the F x F neighbourhood (identity position policy and clamp, spilling into the next
 //     slice as the brute path does) evaluated exactly; fit and finalize as peak_kernel.
-// A slice with more than kRefineCap candidates (a flat, noise-dominated screen) keeps
-// the screen's arg-max.
+// A slice with more than kRefineCap candidates (a flat, noise-dominated screen) or
+// without a finite screen maximum is swept whole: every shift of the window is
+// evaluated exactly, box by box, so its arg-max is the brute policy's as well.
 constexpr int kRefineCap = 512;
-constexpr int kRowList = 256;  // screens of at most this many rows use the row maxima
+constexpr int kSweepRows = 8;  // sweep boxes: kSweepRows x kBoxW shifts (one per thread)
 constexpr int kRefineNT = 256;
+constexpr int kRowList = 256;  // screens of at most this many rows use the row maxima
 
 template <class T>
 __device__ inline T exact_corr(const T* __restrict__ p, const T* __restrict__ r, size2 s, int sx, int sy) {
@@ -352,6 +354,7 @@ __device__ inline T exact_at(size_t g, size_t corr_len, const T* p, const T* r, 
 constexpr int kBoxRows = 14;
 constexpr int kBoxW = 32;
 constexpr int kBoxY = 4;
+static_assert(kSweepRows * kBoxW <= kRefineNT && kSweepRows <= kBoxRows, "sweep box exceeds a box evaluation");
 
 // Row stride of the allocation (elements): the smallest stride >= sx congruent to 9 mod
 // 32.  A box whose conflict-free stride does not fit it falls back to this stride, on
@@ -567,13 +570,8 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
     }
     if (smax) mi = block_min_u32<NT>(mil, si);
     __syncthreads();
-    uint32_t nc = ncand;
-    if (nc > (uint32_t)kRefineCap || mi == 0xffffffffu) {
-        nc = 1;
-        __syncthreads();
-        if (tid == 0) cand[0] = mi == 0xffffffffu ? 0u : mi;
-        __syncthreads();
-    }
+    const bool sweep = ncand > (uint32_t)kRefineCap || mi == 0xffffffffu;
+    const uint32_t nc = sweep ? 0u : ncand;
     // exact values; arg-max with the lowest index on ties.  Candidates whose bounding
     // box fits one box evaluation take the staged path, others one chain each.
     if (tid == 0) {
@@ -596,8 +594,8 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
     const int cbw = bb[2] - bb[0] + 1, cbh = bb[3] - bb[1] + 1;
     const int ex0 = max(0, bb[0] - rad), ey0 = max(0, bb[1] - rad);
     const int ebw = min((int)cross.x - 1, bb[2] + rad) - ex0 + 1, ebh = min((int)cross.y - 1, bb[3] + rad) - ey0 + 1;
-    const bool merged = ebh <= kBoxRows && ebw <= kBoxW && ebw * ebh <= NT;
-    const bool boxed = merged || (cbh <= kBoxRows && cbw <= kBoxW && cbw * cbh <= NT);
+    const bool merged = !sweep && ebh <= kBoxRows && ebw <= kBoxW && ebw * ebh <= NT;
+    const bool boxed = merged || (!sweep && cbh <= kBoxRows && cbw <= kBoxW && cbw * cbh <= NT);
     const int bx0 = merged ? ex0 : bb[0], by0 = merged ? ey0 : bb[1], bw = merged ? ebw : cbw;
     if (boxed) eval_box<T, NT>(ps, rs, s, cross, bx0, by0, bw, merged ? ebh : cbh, prow, boxv);
     T best = lowest<T>();
@@ -611,6 +609,23 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
             best = v;
             bi = i;
         }
+    }
+    if (sweep) {
+        // the whole window, kSweepRows x kBoxW shifts at a time (one per thread)
+        for (int y0 = 0; y0 < (int)cross.y; y0 += kSweepRows)
+            for (int x0 = 0; x0 < (int)cross.x; x0 += kBoxW) {
+                const int w = min(kBoxW, (int)cross.x - x0), h = min(kSweepRows, (int)cross.y - y0);
+                eval_box<T, NT>(ps, rs, s, cross, x0, y0, w, h, prow, boxv);
+                if (tid < w * h) {
+                    const uint32_t i = (uint32_t)(y0 + tid / w) * cross.x + (uint32_t)(x0 + tid % w);
+                    const T v = boxv[tid];
+                    if (v > best || (v == best && i < bi)) {
+                        best = v;
+                        bi = i;
+                    }
+                }
+                __syncthreads();  // the next box stages over these values
+            }
     }
     block_argmax<T, NT>(best, bi, sv, si);
 

@@ -99,12 +99,13 @@ class GpuOffset:
     slice_size, cross_size, src_size: (x, y); policy: CROSS_POLICY_BRUTE,
     CROSS_POLICY_FFT or CROSS_POLICY_BRUTE_MFMA (matrix-core brute correlation);
     s: fitsize 3, 5, 7 or 9; mfma_mode: MFMA_BF16, MFMA_BF16_SPLIT, MFMA_F16 or
-    MFMA_F16_SPLIT, and mfma_refine: exact peak equal to CROSS_POLICY_BRUTE's results
+    MFMA_F16_SPLIT (None: MFMA_F16_SPLIT for uint16 pixels, MFMA_BF16_SPLIT otherwise;
+    the fp16 modes need uint16 pixels), and mfma_refine: exact peak equal to CROSS_POLICY_BRUTE's results
     (False: float screen values, T must be float32); both for CROSS_POLICY_BRUTE_MFMA.
     """
 
     def __init__(self, T, IN, src_size, begins, slice_size, cross_size, batch_size, policy=CROSS_POLICY_BRUTE,
-                 s=3, in_buffers_size=1, mfma_mode=MFMA_F16_SPLIT, mfma_refine=True):
+                 s=3, in_buffers_size=1, mfma_mode=None, mfma_refine=True):
         self.T = np.dtype(T)
         self.IN = np.dtype(IN)
         self.src_size = tuple(int(v) for v in src_size)
@@ -120,6 +121,8 @@ class GpuOffset:
                                             C.size2(cross_size), self.batch_size, self.policy, self.s,
                                             int(in_buffers_size)))
         self._h = h
+        if mfma_mode is None:  # the library's default for this pixel type
+            mfma_mode = MFMA_F16_SPLIT if self.IN == np.uint16 else MFMA_BF16_SPLIT
         if self.policy == CROSS_POLICY_BRUTE_MFMA:
             C.check(C.lib().emida_offset_set_mfma_mode(h, int(mfma_mode)))
             C.check(C.lib().emida_offset_set_mfma_refine(h, 1 if mfma_refine else 0))
@@ -221,6 +224,43 @@ class GpuOffset:
                                                     p(d_neighbors), p(d_offsets), p(d_coefs),
                                                     None if stream is None else ctypes.c_void_p(int(stream))))
 
+    # -- the reference's cross_corr_fft seam and its test accessors (src/gpu_offset.hpp:96-97,303-325)
+    def _dptr(self, fn):
+        p = ctypes.c_void_p()
+        C.check(fn(self._h, ctypes.byref(p)))
+        return p.value
+
+    def get_cu_pic(self) -> int:
+        """Device address of the prepared pattern slices (FFT policy: the (2s.x) x
+        (2s.y) buffers cross_corr_fft reads)."""
+        return self._dptr(C.lib().emida_offset_get_cu_pic)
+
+    def get_cu_temp(self) -> int:
+        """Device address of the reference data (FFT policy: R2C spectra, 2s.y rows of
+        s.x + 1 complex per ROI; brute: prepared reference slices)."""
+        return self._dptr(C.lib().emida_offset_get_cu_temp)
+
+    def get_cu_cross_res(self) -> int:
+        """Device address of the correlation buffers cross_corr_fft writes."""
+        return self._dptr(C.lib().emida_offset_get_cu_cross_res)
+
+    def cross_corr_fft(self):
+        """cross_corr_fft() (src/gpu_offset.hpp:303-316): R2C of the cu_pic buffers,
+        product with conj(cu_temp), unnormalised C2R into the cu_cross_res buffers."""
+        C.check(C.lib().emida_offset_cross_corr_fft(self._h))
+
+    @property
+    def in_stream(self) -> int:
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        C.check(C.lib().emida_offset_get_streams(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value
+
+    @property
+    def out_stream(self) -> int:
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        C.check(C.lib().emida_offset_get_streams(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return b.value
+
     def set_timing(self, enable=True):
         C.check(C.lib().emida_offset_set_timing(self._h, 1 if enable else 0))
 
@@ -260,3 +300,10 @@ def device_query():
     cu = ctypes.c_int()
     C.check(C.lib().emida_device_query(buf, 256, ctypes.byref(cu)))
     return buf.value.decode(), cu.value
+
+
+def device_memory():
+    """(free, total) bytes of device memory on the current device."""
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    C.check(C.lib().emida_device_mem_info(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value

@@ -5,8 +5,9 @@
 // Same constructor and member signatures; errors throw std::runtime_error after
 // printing to stderr, like the reference's CUCH (src/host_helpers.hpp:14-25).
 // Differences: `begins` is copied (the reference keeps the pointer); device
-// memory is released by the destructor; uploads are awaited with
-// wait_for_uploads() instead of cudaStreamSynchronize(in_stream).
+// memory is released by the destructor; in_stream / out_stream are HIP streams
+// (hipStreamSynchronize(offs.in_stream) where the reference calls
+// cudaStreamSynchronize), set by allocate_memory; wait_for_uploads() does the same.
 #pragma once
 
 #include <array>
@@ -18,6 +19,10 @@
 #include <vector>
 
 #include "emida_amd.h"
+
+// HIP's stream handle type (hipStream_t is `struct ihipStream_t*`), declared here so
+// this header needs no HIP include; the two name the same type.
+struct ihipStream_t;
 
 namespace emida {
 
@@ -108,6 +113,10 @@ class gpu_offset {
     }
 
 public:
+    // src/gpu_offset.hpp:96-97: the upload and download streams (set by allocate_memory)
+    ihipStream_t* in_stream = nullptr;
+    ihipStream_t* out_stream = nullptr;
+
     gpu_offset() = default;
     gpu_offset(size2_t src_size, const std::vector<size2_t>* begins, size2_t slice_size, size2_t cross_size,
                esize_t batch_size, cross_policy policy = CROSS_POLICY_BRUTE, int s = 3, size_t in_buffers_size = 1) {
@@ -124,11 +133,16 @@ public:
     }
     gpu_offset(const gpu_offset&) = delete;
     gpu_offset& operator=(const gpu_offset&) = delete;
-    gpu_offset(gpu_offset&& o) noexcept : h_(o.h_), total_(o.total_), neigh_(o.neigh_) { o.h_ = nullptr; }
+    gpu_offset(gpu_offset&& o) noexcept
+        : h_(o.h_), total_(o.total_), neigh_(o.neigh_), in_stream(o.in_stream), out_stream(o.out_stream) {
+        o.h_ = nullptr;
+    }
     gpu_offset& operator=(gpu_offset&& o) noexcept {
         std::swap(h_, o.h_);
         total_ = o.total_;
         neigh_ = o.neigh_;
+        in_stream = o.in_stream;
+        out_stream = o.out_stream;
         return *this;
     }
     ~gpu_offset() {
@@ -148,7 +162,13 @@ public:
     }
 
     // src/gpu_offset.hpp:127
-    void allocate_memory(IN* temp) { detail::check(emida_offset_allocate_memory(h_, temp), "allocate_memory"); }
+    void allocate_memory(IN* temp) {
+        detail::check(emida_offset_allocate_memory(h_, temp), "allocate_memory");
+        void *a = nullptr, *b = nullptr;
+        detail::check(emida_offset_get_streams(h_, &a, &b), "allocate_memory");
+        in_stream = static_cast<ihipStream_t*>(a);
+        out_stream = static_cast<ihipStream_t*>(b);
+    }
 
     // src/gpu_offset.hpp:213
     esize_t transfer_pic_to_device_async(IN* pic) {
@@ -179,8 +199,46 @@ public:
         return unpack(o, cf);
     }
 
+    // src/gpu_offset.hpp:303-316: R2C of the get_cu_pic buffers, product with the
+    // conjugate get_cu_temp spectra, unnormalised C2R (FFT policy; synchronous)
+    void cross_corr_fft() const { detail::check(emida_offset_cross_corr_fft(h_), "cross_corr_fft"); }
+
+    // src/gpu_offset.hpp:318-325 (device pointers, for tests)
+    T* get_cu_pic() {
+        void* p = nullptr;
+        detail::check(emida_offset_get_cu_pic(h_, &p), "get_cu_pic");
+        return static_cast<T*>(p);
+    }
+    T* get_cu_temp() {
+        void* p = nullptr;
+        detail::check(emida_offset_get_cu_temp(h_, &p), "get_cu_temp");
+        return static_cast<T*>(p);
+    }
+    // the buffers cross_corr_fft writes (the reference's private cu_cross_res_)
+    T* get_cu_cross_res() {
+        void* p = nullptr;
+        detail::check(emida_offset_get_cu_cross_res(h_, &p), "get_cu_cross_res");
+        return static_cast<T*>(p);
+    }
+
     emida_offset* handle() { return h_; }
 };
+
+// copy_to_device / copy_from_device (src/host_helpers.hpp:49-72), synchronous
+template <typename T>
+inline void copy_to_device(const std::vector<T>& v, T* dst) {
+    detail::check(emida_memcpy_h2d(dst, v.data(), v.size() * sizeof(T)), "copy_to_device");
+}
+template <typename T>
+inline void copy_to_device(const T* src, size_t n, T* dst) {
+    detail::check(emida_memcpy_h2d(dst, src, n * sizeof(T)), "copy_to_device");
+}
+template <typename T>
+inline std::vector<T> device_to_vector(const T* src, size_t n) {
+    std::vector<T> v(n);
+    detail::check(emida_memcpy_d2h(v.data(), src, n * sizeof(T)), "device_to_vector");
+    return v;
+}
 
 // get_offset<T>(pic, temp, size, cross_size, policy), src/get_offset.hpp:8-14: one ROI
 // covering the whole size.x x size.y picture at {0, 0}, batch 1.

@@ -53,7 +53,10 @@ typedef enum {
 /* Operand precision of EMIDA_CROSS_POLICY_BRUTE_MFMA.  *_SPLIT modes store every
  * prepared value as hi + lo halves and accumulate hi*hi + hi*lo + lo*hi (relative
  * representation error 2^-16 for bf16, 2^-22 for fp16).  fp16 operands are the
- * prepared values halved, so any |value| < 131008 fits (all uint16 inputs). */
+ * prepared values halved, so any |value| < 131008 fits: every uint16 input does, so
+ * the fp16 modes are accepted for IN = U16 only (emida_offset_set_mfma_mode returns
+ * EMIDA_ERR_UNSUPPORTED otherwise).  Default: F16_SPLIT for U16 patterns, BF16_SPLIT
+ * for F32 and F64 patterns. */
 typedef enum {
     EMIDA_MFMA_BF16 = 0,
     EMIDA_MFMA_BF16_SPLIT = 1,
@@ -84,7 +87,7 @@ typedef struct {
     uint32_t data_index_bytes; /* sizeof(data_index<T>): 8 (F32) or 16 (F64) */
     emida_size2 cross_size;    /* effective correlation size (FFT forces 2s-1) */
     emida_size2 corr_buffer;   /* arg-max domain: cross_size (brute) or 2s (FFT) */
-    uint32_t fft_size;         /* padded FFT side M (0 for brute) */
+    uint32_t fft_size;         /* padded FFT side M of the x axis: the power of two >= 2 s.x (0 for brute) */
     uint64_t device_bytes;     /* device memory held by the object */
 } emida_offset_info;
 
@@ -151,8 +154,37 @@ int emida_offset_get_offset(emida_offset* h, const void* pic, double* offsets, d
 int emida_offset_process_device(emida_offset* h, const void* d_pics, uint32_t n_patterns, void* d_maxes,
                                 void* d_neighbors, double* d_offsets, double* d_coefs, void* stream);
 
-/* Operand precision of the BRUTE_MFMA policy (default EMIDA_MFMA_F16_SPLIT); call
- * before emida_offset_allocate_memory.  No reference counterpart. */
+/* get_cu_pic(), src/gpu_offset.hpp:318-321: device pointer to the prepared pattern
+ * slices.  FFT policy: the input of emida_offset_cross_corr_fft, total_slices buffers
+ * of (2 s.x) x (2 s.y) T (rows of 2 s.x), zeroed when first requested; the pipeline
+ * itself keeps its prepared slices on chip and does not write them.  BRUTE (and the
+ * refined BRUTE_MFMA) policy: the s.x x s.y T slices get_offset_core prepares.
+ * The FFT policy's seam needs 2s to be a power of two on both axes
+ * (EMIDA_ERR_UNSUPPORTED otherwise). */
+int emida_offset_get_cu_pic(emida_offset* h, void** dptr);
+/* get_cu_temp(), src/gpu_offset.hpp:322-325.  FFT policy: the ROIs' reference spectra
+ * in cuFFT's R2C layout, 2 s.y rows of s.x + 1 complex T per ROI (the bytes of the
+ * reference's (2 s.x + 2) x 2 s.y in-place buffer), the unnormalised R2C of the
+ * zero-padded prepared reference slices when allocate_memory had a reference.  BRUTE
+ * (and refined BRUTE_MFMA): the prepared reference slices, s.x x s.y T each. */
+int emida_offset_get_cu_temp(emida_offset* h, void** dptr);
+/* cross_corr_fft(), src/gpu_offset.hpp:303-316, FFT policy: R2C of every cu_pic buffer,
+ * product with the conjugate of its ROI's cu_temp spectrum (src/hadamard.cu:39-40),
+ * unnormalised C2R (the column transform first, then each row's C2R, which ignores
+ * the imaginary parts of its DC and Nyquist terms) into the buffers of
+ * emida_offset_get_cu_cross_res; synchronous. */
+int emida_offset_cross_corr_fft(emida_offset* h);
+/* The correlation buffers cross_corr_fft writes (the reference's cu_cross_res_):
+ * total_slices x (2 s.x) x (2 s.y) T.  No public accessor in the reference. */
+int emida_offset_get_cu_cross_res(emida_offset* h, void** dptr);
+/* The object's upload and download streams as hipStream_t values: the reference's
+ * public in_stream / out_stream members (src/gpu_offset.hpp:96-97), made by
+ * allocate_memory. */
+int emida_offset_get_streams(emida_offset* h, void** in_stream, void** out_stream);
+
+/* Operand precision of the BRUTE_MFMA policy (default EMIDA_MFMA_F16_SPLIT for U16
+ * patterns, EMIDA_MFMA_BF16_SPLIT for F32/F64 patterns; the fp16 modes need IN = U16);
+ * call before emida_offset_allocate_memory.  No reference counterpart. */
 int emida_offset_set_mfma_mode(emida_offset* h, emida_mfma_mode mode);
 
 /* Exact peak for the BRUTE_MFMA policy (default on): the matrix-core correlation only
@@ -183,6 +215,8 @@ int emida_memset_device(void* dst, int value, size_t bytes);
 int emida_device_synchronize(void);
 /* Device name and compute-unit count of the current device. */
 int emida_device_query(char* name, size_t name_len, int* cu_count);
+/* Free and total device memory of the current device (hipMemGetInfo). */
+int emida_device_mem_info(size_t* free_bytes, size_t* total_bytes);
 
 /* ------------------------------------------------------------------ kernel-level seam
  * Counterparts of the launchers in src/kernels.cuh:56-125 (the reference's tests call
@@ -209,8 +243,8 @@ int emida_run_cross_corr(emida_type T, const void* pics, const void* ref, void* 
 
 /* cross_corr_fft(), src/gpu_offset.hpp:303-316 (R2C, hadamard, C2R; unnormalised):
  * prepared size.x x size.y slices -> (2 size.x) x (2 size.y) circular correlation
- * buffers, batch x ref_slices of them.  size must be square with a side of 2..128
- * (F32) or 2..64 (F64). */
+ * buffers (rows of 2 size.x values), batch x ref_slices of them.  Sides 2..256 per
+ * axis, square or not. */
 int emida_run_cross_corr_fft(emida_type T, const void* pics, const void* ref, void* res, emida_size2 size,
                              uint32_t ref_slices, uint32_t batch_size, void* stream);
 
@@ -248,9 +282,10 @@ int emida_run_extract_neighbors(emida_type T, emida_cross_policy pos_policy, con
 int emida_run_subpixel_max(emida_type T, const void* neighbors, int s, uint32_t n, double* offsets,
                            double* coefs, void* stream);
 
-/* Largest slice side the FFT policy accepts for element type T (any side from 2 up
- * to this; the transform is padded to the next power of two >= 2s, with the same
- * correlation values).  No counterpart in the reference, where cuFFT plans any size. */
+/* Largest slice side (per axis) the FFT policy accepts for element type T: 256 for
+ * both (any side from 2 up to this, square or not; each axis' transform is padded to
+ * the next power of two >= 2s, with the same correlation values).  No counterpart in
+ * the reference, where cuFFT plans any size. */
 int emida_fft_max_slice(emida_type T, uint32_t* side);
 
 /* get_lstsq_matrix(s), src/subpixel_max.cpp:5-20: (A^T A)^-1 A^T as 6 x s^2 doubles. */

@@ -332,10 +332,13 @@ void subpixel(const T* neigh, int s, size_t n, double* offsets, double* coefs) {
 // ---------------------------------------------------------------- whole path
 // src/gpu_offset.hpp:100-125 (sizes), :127-211 (reference preparation),
 // :234-278 (one batch), :280-301 (finalize).
+// peak_stats (optional, 3 per slice): the largest correlation value, the second
+// largest (equal to it on a tie) and the largest magnitude - how well the integer
+// peak is separated, for the tests' and the benchmark's conditioning criterion.
 template <class T, class IN, class S>
 int get_offset(int policy, const IN* pics, const IN* temp, sz2 src, const uint32_t* begins, uint32_t nb,
                sz2 slice, sz2 cross, uint32_t batch, int fit, double* offsets, double* coefs,
-               data_index<T>* maxes_out, T* neigh_out) {
+               data_index<T>* maxes_out, T* neigh_out, double* peak_stats) {
     if (fit != 3 && fit != 5 && fit != 7 && fit != 9) return -1;
     const uint32_t total = nb * batch;
     if (policy == POLICY_FFT) cross = sz2{2 * slice.x - 1, 2 * slice.y - 1};
@@ -356,6 +359,24 @@ int get_offset(int policy, const IN* pics, const IN* temp, sz2 src, const uint32
         cross_corr<T>(pic.data(), ref.data(), corr.data(), slice, cross, nb, batch);
     std::vector<data_index<T>> mx(total);
     maxarg<T>(corr.data(), buf.area(), total, mx.data());
+    if (peak_stats)
+        for (uint32_t i = 0; i < total; ++i) {
+            const T* d = corr.data() + (size_t)i * buf.area();
+            double m1 = -HUGE_VAL, m2 = -HUGE_VAL, amax = 0;
+            for (uint32_t k = 0; k < buf.area(); ++k) {
+                const double v = (double)d[k];
+                if (v > m1) {
+                    m2 = m1;
+                    m1 = v;
+                } else if (v > m2) {
+                    m2 = v;
+                }
+                amax = std::max(amax, std::fabs(v));
+            }
+            peak_stats[3 * (size_t)i] = m1;
+            peak_stats[3 * (size_t)i + 1] = m2;
+            peak_stats[3 * (size_t)i + 2] = amax;
+        }
     std::vector<T> nbh((size_t)total * fit * fit);
     extract<T>(corr.data(), corr.size(), mx.data(), nbh.data(), fit, cross, total, policy);
     std::vector<double> sub(2 * total);
@@ -484,11 +505,13 @@ int or_subpixel(int T, const void* neigh, int s, size_t n, double* offsets, doub
 
 int or_get_offset(int T, int IN, int policy, const void* pics, const void* temp, uint32_t srcx, uint32_t srcy,
                   const uint32_t* begins, uint32_t nb, uint32_t slx, uint32_t sly, uint32_t crx, uint32_t cry,
-                  uint32_t batch, int fit, double* offsets, double* coefs, void* maxes_out, void* neigh_out) {
+                  uint32_t batch, int fit, double* offsets, double* coefs, void* maxes_out, void* neigh_out,
+                  double* peak_stats) {
     sz2 src{srcx, srcy}, sl{slx, sly}, cr{crx, cry};
 #define GO(TT, INT, ST)                                                                                  \
     return get_offset<TT, INT, ST>(policy, (const INT*)pics, (const INT*)temp, src, begins, nb, sl, cr, \
-                                   batch, fit, offsets, coefs, (data_index<TT>*)maxes_out, (TT*)neigh_out)
+                                   batch, fit, offsets, coefs, (data_index<TT>*)maxes_out, (TT*)neigh_out, \
+                                   peak_stats)
     if (T == T_F64) {
         if (IN == T_U16) GO(double, uint16_t, uint32_t);
         if (IN == T_F64) GO(double, double, double);

@@ -71,7 +71,7 @@ def lib():
         L.or_lstsq_matrix.argtypes = [i32, dp]
         L.or_subpixel.argtypes = [i32, vp, i32, sz, vp, vp]
         L.or_get_offset.argtypes = [i32, i32, i32, vp, vp, u32, u32, vp, u32, u32, u32, u32, u32, u32, i32,
-                                    vp, vp, vp, vp]
+                                    vp, vp, vp, vp, vp]
         _LIB = L
     return _LIB
 
@@ -174,10 +174,12 @@ def subpixel(neigh, s, n=None):
 
 
 def get_offset(pics, temp, src, begins, slice_size, cross_size, batch, fitsize=3, policy=POLICY_BRUTE,
-               dtype=np.float64, return_intermediates=False):
+               dtype=np.float64, return_intermediates=False, return_stats=False):
     """Whole-path oracle: gpu_offset<T,IN> construct + allocate_memory(temp) +
     get_offset(pics) (src/gpu_offset.hpp:100-301). Returns offsets (n,2) and
-    coefs (n,6) in double; optionally the arg-max records and neighbourhoods."""
+    coefs (n,6) in double; optionally the arg-max records and neighbourhoods, and
+    (return_stats) per slice the largest correlation value, the second largest and
+    the largest magnitude, (n,3)."""
     pics = np.ascontiguousarray(pics)
     temp = np.ascontiguousarray(temp, dtype=pics.dtype)
     b = _begins(begins)
@@ -186,10 +188,35 @@ def get_offset(pics, temp, src, begins, slice_size, cross_size, batch, fitsize=3
     coefs = np.zeros((total, 6))
     maxes = np.zeros(total, data_index_dtype(dtype))
     neigh = np.zeros(total * fitsize * fitsize, dtype)
+    stats = np.zeros((total, 3)) if return_stats else None
     rc = lib().or_get_offset(_TCODE[np.dtype(dtype)], _TCODE[pics.dtype], policy, _p(pics), _p(temp), src[0],
                              src[1], _p(b), len(b), slice_size[0], slice_size[1], cross_size[0], cross_size[1],
-                             batch, fitsize, _p(offs), _p(coefs), _p(maxes), _p(neigh))
+                             batch, fitsize, _p(offs), _p(coefs), _p(maxes), _p(neigh), _p(stats))
     _check(rc, "get_offset")
+    out = (offs, coefs)
     if return_intermediates:
-        return offs, coefs, maxes, neigh
-    return offs, coefs
+        out += (maxes, neigh)
+    if return_stats:
+        out += (stats,)
+    return out
+
+
+# Relative peak margin above which float rounding cannot move the integer peak (the
+# float FFT's error stays far below 1e-4 of the largest magnitude).
+FLOAT_MARGIN = 1e-4
+
+
+def well_conditioned(coefs, fit, stats, rel_margin=FLOAT_MARGIN):
+    """The ROIs on which an offset error against the FP64 oracle is measured: the FP64
+    peak beats the runner-up by more than rel_margin x the largest magnitude (the
+    integer peak is fixed) and the FP64 quadratic fit has its maximum inside the
+    F x F neighbourhood (elsewhere the fitted position is an extrapolation whose
+    error grows without bound as the curvature vanishes, in any precision)."""
+    b, c, d, e, f = (coefs[:, i] for i in range(1, 6))
+    det = 4 * f * d - e * e
+    with np.errstate(divide="ignore", invalid="ignore"):
+        py = (b * e - 2 * c * d) / det
+        px = (-e * py - b) / (2 * d)
+    posed = (d < 0) & (det > 0) & (px >= 0) & (px <= fit - 1) & (py >= 0) & (py <= fit - 1)
+    margin = (stats[:, 0] - stats[:, 1]) > rel_margin * stats[:, 2]
+    return posed & margin

@@ -226,25 +226,27 @@ def test_cross_corr_fft_128_float_vs_oracle(s, ref_slices, batch):
 
 
 def _rocfft_corr(pics, ref, s, ref_slices, batch):
-    """The reference's R2C (2s x 2s, zero-padded), p * conj(r) and unnormalised C2R
+    """The reference's R2C (2s.x x 2s.y, zero-padded), p * conj(r) and unnormalised C2R
     (src/gpu_offset.hpp:303-316, src/cufft_helpers.hpp:105-155) with rocFFT through
-    the system hipFFT library, in the HIP runtime the library under test uses."""
+    the system hipFFT library, in the HIP runtime the library under test uses.  The
+    plan's dimensions are (rows, columns) = (2 s.y, 2 s.x)."""
     import ctypes
     from emida_amd.gpu_offset import DeviceBuffer
     fft = ctypes.CDLL("libhipfft.so")
     dt = pics.dtype
     double = dt == np.float64
     cdt = np.complex128 if double else np.complex64
-    n, m = ref_slices * batch, 2 * s
-    nc = m * (m // 2 + 1)
-    P = np.zeros((n, m, m), dt)
-    P[:, :s, :s] = pics.reshape(n, s, s)
-    R = np.zeros((n, m, m), dt)
-    R[:, :s, :s] = np.tile(ref.reshape(ref_slices, s, s), (batch, 1, 1))  # slice i uses ref i % ref_slices
+    sx, sy = (s, s) if np.isscalar(s) else s
+    n, mx, my = ref_slices * batch, 2 * sx, 2 * sy
+    nc = my * (mx // 2 + 1)
+    P = np.zeros((n, my, mx), dt)
+    P[:, :sy, :sx] = pics.reshape(n, sy, sx)
+    R = np.zeros((n, my, mx), dt)
+    R[:, :sy, :sx] = np.tile(ref.reshape(ref_slices, sy, sx), (batch, 1, 1))  # slice i uses ref i % ref_slices
 
     def plan(kind):
         h = ctypes.c_void_p()
-        dims = (ctypes.c_int * 2)(m, m)
+        dims = (ctypes.c_int * 2)(my, mx)
         assert fft.hipfftPlanMany(ctypes.byref(h), 2, dims, None, 1, 0, None, 1, 0, kind, n) == 0
         return h
 
@@ -257,29 +259,73 @@ def _rocfft_corr(pics, ref, s, ref_slices, batch):
         assert fwd(r2c, d_in.ptr, d_out.ptr) == 0
         spectra.append(d_out.download(cdt))
     H = (spectra[0] * np.conj(spectra[1])).astype(cdt)
-    d_h, d_c = DeviceBuffer.from_array(H), DeviceBuffer(n * m * m * dt.itemsize)
+    d_h, d_c = DeviceBuffer.from_array(H), DeviceBuffer(n * mx * my * dt.itemsize)
     assert inv(c2r, d_h.ptr, d_c.ptr) == 0
-    out = d_c.download(dt).reshape(n, m, m)
+    out = d_c.download(dt).reshape(n, my, mx)
     fft.hipfftDestroy(r2c)
     fft.hipfftDestroy(c2r)
     return out
 
 
-@pytest.mark.parametrize("s", [64, 20, 33, 128, 96])
+@pytest.mark.parametrize("s", [64, 20, 33, 128, 96, 160, (64, 48), (20, 100), (130, 9)])
 @pytest.mark.parametrize("dt,tol", [(np.float64, 1e-12), (np.float32, 3e-6)])
 def test_cross_corr_fft_vs_rocfft(s, dt, tol):
-    # cross-check of the FFT correlation against rocFFT (used in this test only)
-    if dt == np.float64 and s > 64:
-        pytest.skip("double FFT slices are at most 64 wide")
-    rng = np.random.default_rng(31 + s)
+    # cross-check of the FFT correlation against rocFFT (used in this test only),
+    # every path: one kernel, the 256-point float kernels and the three-kernel path
+    # (double sides above 64, sides above 128, non-square slices)
+    sx, sy = (s, s) if np.isscalar(s) else s
+    rng = np.random.default_rng(31 + sx + 7 * sy)
     ref_slices, batch = 3, 2
     n = ref_slices * batch
-    pics = rng.standard_normal(s * s * n).astype(dt)
-    ref = rng.standard_normal(s * s * ref_slices).astype(dt)
-    got = K.run_cross_corr_fft(pics, ref, (s, s), ref_slices, batch).reshape(n, 2 * s, 2 * s).astype(np.float64)
-    exp = _rocfft_corr(pics, ref, s, ref_slices, batch).astype(np.float64)
+    pics = rng.standard_normal(sx * sy * n).astype(dt)
+    ref = rng.standard_normal(sx * sy * ref_slices).astype(dt)
+    got = K.run_cross_corr_fft(pics, ref, (sx, sy), ref_slices, batch).reshape(n, 2 * sy, 2 * sx)
+    exp = _rocfft_corr(pics, ref, (sx, sy), ref_slices, batch).astype(np.float64)
     scale = np.abs(exp).reshape(n, -1).max(axis=1)[:, None, None]
-    assert (np.abs(got - exp) / scale).max() < tol * np.log2(2 * s)
+    assert (np.abs(got.astype(np.float64) - exp) / scale).max() < tol * np.log2(2 * max(sx, sy))
+
+
+def _numpy_fft_corr(pics, ref, size, ref_slices, batch):
+    """(2 s.x) x (2 s.y) unnormalised circular correlation of zero-padded slices by
+    numpy's FFT in double: irfft2(rfft2(p) conj(rfft2(r))) x 4 s.x s.y."""
+    sx, sy = size
+    p = pics.astype(np.float64).reshape(batch, ref_slices, sy, sx)
+    r = ref.astype(np.float64).reshape(1, ref_slices, sy, sx)
+    P = np.fft.rfft2(p, s=(2 * sy, 2 * sx))
+    R = np.fft.rfft2(r, s=(2 * sy, 2 * sx))
+    return (np.fft.irfft2(P * np.conj(R), s=(2 * sy, 2 * sx)) * (4 * sx * sy)).reshape(batch * ref_slices, -1)
+
+
+@pytest.mark.parametrize("size,ref_slices,batch", [
+    ((96, 96), 2, 2), ((128, 128), 2, 3), ((65, 65), 1, 2),       # double: 256-point transforms
+    ((160, 160), 2, 2), ((256, 256), 1, 2), ((129, 129), 1, 1),  # 512-point transforms
+    ((64, 48), 3, 2), ((48, 64), 2, 3), ((20, 100), 2, 2), ((200, 3), 1, 3), ((2, 2), 1, 1)])
+@pytest.mark.parametrize("dt,tol", [(np.float64, 1e-12), (np.float32, 3e-6)])
+def test_cross_corr_fft_three_kernel_path_vs_numpy(size, ref_slices, batch, dt, tol):
+    # the shapes of the three-kernel path (and a few single-kernel ones beside them)
+    # against numpy's double FFT of the same zero-padded slices
+    rng = np.random.default_rng(size[0] * 3 + size[1] + batch)
+    a = size[0] * size[1]
+    pics = rng.standard_normal(a * ref_slices * batch).astype(dt)
+    ref = rng.standard_normal(a * ref_slices).astype(dt)
+    got = K.run_cross_corr_fft(pics, ref, size, ref_slices, batch).reshape(ref_slices * batch, -1)
+    exp = _numpy_fft_corr(pics, ref, size, ref_slices, batch)
+    scale = np.abs(exp).max(axis=1, keepdims=True)
+    assert (np.abs(got.astype(np.float64) - exp) / scale).max() < tol * np.log2(2 * max(size))
+
+
+@pytest.mark.parametrize("size", [(20, 12), (12, 20), (64, 48), (33, 5)])
+@pytest.mark.parametrize("dt,tol", [(np.float64, 1e-12), (np.float32, 3e-6)])
+def test_cross_corr_fft_non_square_vs_oracle(size, dt, tol):
+    rng = np.random.default_rng(size[0] + 11 * size[1])
+    ref_slices, batch = 2, 2
+    a = size[0] * size[1]
+    pics = rng.standard_normal(a * ref_slices * batch).astype(dt)
+    ref = rng.standard_normal(a * ref_slices).astype(dt)
+    got = K.run_cross_corr_fft(pics, ref, size, ref_slices, batch).reshape(ref_slices * batch, -1)
+    exp = O.fft_corr(pics, ref, size, ref_slices, batch).reshape(ref_slices * batch, -1).astype(np.float64)
+    scale = np.abs(exp).max(axis=1, keepdims=True)
+    assert (np.abs(got.astype(np.float64) - exp) / scale).max() < tol * np.log2(2 * max(size))
 
 
 def test_cross_corr_fft_smoke_4x4():  # test/cross_corr_fft_test.cpp:8-42 (same inputs, with an assertion)

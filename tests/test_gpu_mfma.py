@@ -246,3 +246,125 @@ def test_precision_form_workload_subset_vs_fp64_oracle():
     for mode in (emida_amd.MFMA_BF16_SPLIT, emida_amd.MFMA_F16_SPLIT):
         offs = _run(ps, begins, (128, 128), (255, 255), 7, 1, mode=mode, refine=False)[0]
         assert np.abs(offs - eo)[ok].max() < 0.01
+
+
+# ------------------------------------------------------------------ flat screens
+def _tau(p, r, mode, size, t_double):
+    """The refinement's candidate bound tau = tol_rel ||p|| ||r|| + tol_abs (||p|| + ||r||)
+    for mode (the arithmetic of mfma_refine_bounds, restated)."""
+    u = 2.0 ** -11 if mode in (emida_amd.MFMA_F16, emida_amd.MFMA_F16_SPLIT) else 2.0 ** -8
+    split = mode in (emida_amd.MFMA_BF16_SPLIT, emida_amd.MFMA_F16_SPLIT)
+    n = size[0] * size[1]
+    e_op = (3.1 * u * u if split else 2 * u + u * u) + (2.01 * 2.0 ** -24 if t_double else 0.0)
+    nt, u32 = n * (3 if split else 1), 2.0 ** -24
+    g_acc = nt * u32 / (1 - nt * u32)
+    ut = 2.0 ** -53 if t_double else u32
+    g_t = n * ut / (1 - n * ut)
+    h = 2.1 * 2.0 ** -24 if u == 2.0 ** -11 else 0.0
+    tol_rel = 2.02 * (e_op * (1 + g_acc) + g_acc * (1 + u) ** 2 + g_t)
+    tol_abs = 2.02 * h * np.sqrt(n) * (1 + g_acc) * (2 if split else 1)
+    npn, nrn = np.sqrt((p.astype(np.float64) ** 2).sum()), np.sqrt((r.astype(np.float64) ** 2).sum())
+    return tol_rel * npn * nrn + tol_abs * (npn + nrn)
+
+
+def _plateau_patterns(batch):
+    """256 x 128 uint16 patterns of two 128 x 128 ROIs.  The deformed patterns hold one
+    bright pixel on a dark ROI, the reference a smooth blob: the correlation is the
+    windowed blob, shifted, whose broad top leaves thousands of shifts within a plain
+    half-type screen's rounding bound of the maximum."""
+    yy, xx = np.mgrid[0:128, 0:256].astype(float)
+    ref = 500 + 40000 * np.exp(-((xx - 60) ** 2 + (yy - 70) ** 2) / (2 * 22.0 ** 2))
+    ref += 30000 * np.exp(-((xx - 200) ** 2 + (yy - 50) ** 2) / (2 * 28.0 ** 2))
+    ref = np.clip(ref, 0, 65535).astype(np.uint16)
+    pics = np.zeros((batch, 128, 256), np.uint16)
+    for b in range(batch):
+        pics[b, 50 + 7 * b, 70 - 5 * b] = 60000
+        pics[b, 40 + 3 * b, 128 + 80 + 4 * b] = 50000
+
+    class P:
+        pass
+
+    ps = P()
+    ps.ref, ps.pics, ps.size = ref, pics, (256, 128)
+    return ps, [(0, 0), (128, 0)]
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+@pytest.mark.parametrize("mode", [emida_amd.MFMA_BF16])
+def test_refined_equals_brute_policy_beyond_candidate_cap(T, mode):
+    # more than 512 candidates: the refinement sweeps the whole window exactly, so the
+    # arg-max is still the brute policy's (it used to keep the screen's arg-max)
+    ps, begins = _plateau_patterns(2)
+    size, cross = (128, 128), (255, 255)
+    # the candidates the screen leaves, counted as the refinement counts them
+    hx = O.hann(128, np.float32)
+    sp = O.sums(ps.pics.ravel(), begins, ps.size, size, 2)
+    sr = O.sums(ps.ref.ravel(), begins, ps.size, size, 1)
+    pf = O.prepare(ps.pics.ravel(), hx, hx, sp, begins, ps.size, size, None, 2).astype(np.float32)
+    rf = O.prepare(ps.ref.ravel(), hx, hx, sr, begins, ps.size, size, None, 1).astype(np.float32)
+    screen = K.run_cross_corr_mfma(pf, rf, size, cross, 2, 2, mode).reshape(4, -1).astype(np.float64)
+    hd = O.hann(128)
+    pd = O.prepare(ps.pics.ravel(), hd, hd, sp, begins, ps.size, size, None, 2).reshape(4, -1)
+    rd = O.prepare(ps.ref.ravel(), hd, hd, sr, begins, ps.size, size, None, 1).reshape(2, -1)
+    for sn in range(4):
+        tau = _tau(pd[sn], rd[sn % 2], mode, size, T == np.float64)
+        n_cand = int((screen[sn] >= screen[sn].max() - tau).sum())
+        assert n_cand > 512, (sn, n_cand)
+    got = _run(ps, begins, size, cross, 5, 2, T=T, mode=mode)
+    exp = _run(ps, begins, size, cross, 5, 2, T=T, policy=emida_amd.CROSS_POLICY_BRUTE)
+    for g, e, what in zip(got, exp, ("offsets", "coefs", "maxes", "neighbourhoods")):
+        np.testing.assert_array_equal(g, e, err_msg=what)
+
+
+@pytest.mark.parametrize("mode", [emida_amd.MFMA_BF16, emida_amd.MFMA_BF16_SPLIT])
+@pytest.mark.parametrize("data", ["plateau", "stripes"])
+def test_screen_only_row_maxima_equal_full_scan(mode, data):
+    # the screen-only pipeline finds the arg-max from the matrix-core kernel's per-row
+    # maxima; the kernel seam's full scan of the same screen (maxarg_reduce, lowest
+    # index on ties) and gather must give the same records and neighbourhoods bit for
+    # bit, on inputs with many ties
+    if data == "plateau":
+        ps, begins = _plateau_patterns(2)
+        size, cross, batch = (128, 128), (255, 255), 2
+    else:
+        ps, begins = _spread_patterns(2)
+        size, cross, batch = (128, 128), (255, 255), 2
+    nb = len(begins)
+    hx = O.hann(size[0], np.float32)
+    sp = O.sums(ps.pics[:batch].ravel(), begins, ps.size, size, batch)
+    sr = O.sums(ps.ref.ravel(), begins, ps.size, size, 1)
+    pf = O.prepare(ps.pics[:batch].ravel(), hx, hx, sp, begins, ps.size, size, None, batch).astype(np.float32)
+    rf = O.prepare(ps.ref.ravel(), hx, hx, sr, begins, ps.size, size, None, 1).astype(np.float32)
+    screen = K.run_cross_corr_mfma(pf, rf, size, cross, nb, batch, mode)
+    full = K.run_maxarg_reduce(screen, cross, nb * batch)
+    neigh = K.run_extract_neighbors(screen, full, 7, cross, nb * batch)
+    _, _, maxes, got_neigh = _run(ps, begins, size, cross, 7, batch, mode=mode, refine=False)
+    np.testing.assert_array_equal(maxes["index"], full["index"])
+    np.testing.assert_array_equal(maxes["data"], full["data"])
+    np.testing.assert_array_equal(got_neigh, neigh)
+
+
+def test_matrix_core_objects_release_their_memory():
+    # create / allocate / destroy in a loop: every device buffer of the matrix-core
+    # policy is given back.  8 x 128 slices with a 15 x 255 window and 2048 slices per
+    # batch give per-row screen maxima of 2 MiB, a buffer the destructor once missed
+    ps = synth.PatternSet("mfma-leak", 256, 1, t_range=2.0)
+    begins = [(16 * i, 64) for i in range(16)]
+
+    def once():
+        with emida_amd.GpuOffset(np.float64, np.uint16, ps.size, begins, (8, 128), (15, 255), 128,
+                                 emida_amd.CROSS_POLICY_BRUTE_MFMA, 3) as g:
+            g.allocate_memory(ps.ref)
+            return g.device_bytes
+
+    once()
+    emida_amd.synchronize()
+    free0, _ = emida_amd.device_memory()
+    held = 0
+    for _ in range(30):
+        held = once()
+    emida_amd.synchronize()
+    free1, _ = emida_amd.device_memory()
+    assert held > 32 << 20
+    # 30 objects; a 2 MiB buffer left behind by each would take 60 MiB
+    assert free0 - free1 < 16 << 20, (free0 - free1, held)

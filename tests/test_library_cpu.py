@@ -56,10 +56,8 @@ def test_create_valid_configurations_without_gpu():
     (dict(batch_size=0), C.ERR_ARG),
     (dict(begins=[]), C.ERR_ARG),
     (dict(IN=np.int32), C.ERR_ARG),
-    (dict(policy=emida_amd.CROSS_POLICY_FFT, slice_size=(65, 65), src_size=(300, 300)), C.ERR_UNSUPPORTED),
-    (dict(policy=emida_amd.CROSS_POLICY_FFT, slice_size=(32, 16)), C.ERR_UNSUPPORTED),
-    (dict(policy=emida_amd.CROSS_POLICY_FFT, slice_size=(128, 128)), C.ERR_UNSUPPORTED),  # double: <= 64
-    (dict(policy=emida_amd.CROSS_POLICY_FFT, T=np.float32, slice_size=(256, 256), src_size=(300, 300)),
+    (dict(policy=emida_amd.CROSS_POLICY_FFT, slice_size=(257, 16), src_size=(300, 300)), C.ERR_UNSUPPORTED),
+    (dict(policy=emida_amd.CROSS_POLICY_FFT, T=np.float32, slice_size=(16, 257), src_size=(300, 300)),
      C.ERR_UNSUPPORTED),
     (dict(policy=emida_amd.CROSS_POLICY_BRUTE_MFMA, mfma_refine=False), C.ERR_ARG),  # float screen, T double
     (dict(policy=emida_amd.CROSS_POLICY_BRUTE_MFMA, T=np.float32, slice_size=(36, 32)), C.ERR_UNSUPPORTED),
@@ -73,14 +71,20 @@ def test_create_rejects_invalid_arguments(kw, code):
 
 
 def test_fft_limits():
-    assert emida_amd.fft_max_slice(np.float32) == 128 and emida_amd.fft_max_slice(np.float64) == 64
+    assert emida_amd.fft_max_slice(np.float32) == 256 and emida_amd.fft_max_slice(np.float64) == 256
     g = _create(policy=emida_amd.CROSS_POLICY_FFT, T=np.float32, slice_size=(128, 128), cross_size=(1, 1))
     assert g.cross_size == (255, 255) and g.fft_size == 256
     # any side: transforms padded to the next power of two >= 2s, reference buffer 2s
     for T, side, m in [(np.float64, 48, 128), (np.float64, 3, 8), (np.float32, 96, 256), (np.float32, 65, 256),
-                       (np.float64, 64, 128), (np.float32, 33, 128)]:
+                       (np.float64, 64, 128), (np.float32, 33, 128), (np.float64, 128, 256), (np.float64, 65, 256),
+                       (np.float64, 256, 512), (np.float32, 129, 512), (np.float32, 256, 512)]:
         g = _create(policy=emida_amd.CROSS_POLICY_FFT, T=T, slice_size=(side, side), src_size=(300, 300))
         assert g.fft_size == m and g.corr_buffer == (2 * side, 2 * side) and g.cross_size == (2 * side - 1,) * 2
+    # non-square slices: (2 s.x) x (2 s.y) buffers, fft_size is the x axis' transform
+    for T, sl, m in [(np.float64, (64, 48), 128), (np.float32, (20, 100), 64), (np.float32, (200, 3), 512)]:
+        g = _create(policy=emida_amd.CROSS_POLICY_FFT, T=T, slice_size=sl, src_size=(300, 300))
+        assert g.fft_size == m and g.corr_buffer == (2 * sl[0], 2 * sl[1])
+        assert g.cross_size == (2 * sl[0] - 1, 2 * sl[1] - 1)
 
 
 def test_create_matrix_core_policy_without_gpu():
@@ -89,6 +93,22 @@ def test_create_matrix_core_policy_without_gpu():
         assert g.cross_size == (21, 41) and g.neigh_size == 9 and g.mfma_mode == mode
         g = _create(T=np.float64, policy=emida_amd.CROSS_POLICY_BRUTE_MFMA, mfma_mode=mode)  # refined: double
         assert g.data_index.itemsize == 16
+
+
+def test_fp16_matrix_core_modes_need_uint16_patterns():
+    # a float or double pattern can hold prepared values beyond fp16's range (131008
+    # after the halving), which would put inf into the screen: those patterns default
+    # to bf16 split and an explicit fp16 mode is refused before any device work
+    for IN in (np.float32, np.float64):
+        g = _create(T=np.float64, IN=IN, policy=emida_amd.CROSS_POLICY_BRUTE_MFMA)
+        assert g.mfma_mode == emida_amd.MFMA_BF16_SPLIT
+        for mode in (emida_amd.MFMA_BF16, emida_amd.MFMA_BF16_SPLIT):
+            assert _create(T=np.float64, IN=IN, policy=emida_amd.CROSS_POLICY_BRUTE_MFMA, mfma_mode=mode).mfma_mode == mode
+        for mode in (emida_amd.MFMA_F16, emida_amd.MFMA_F16_SPLIT):
+            with pytest.raises(emida_amd.EmidaError) as e:
+                _create(T=np.float64, IN=IN, policy=emida_amd.CROSS_POLICY_BRUTE_MFMA, mfma_mode=mode)
+            assert e.value.code == C.ERR_UNSUPPORTED and "bf16" in str(e.value)
+    assert _create(T=np.float64, policy=emida_amd.CROSS_POLICY_BRUTE_MFMA).mfma_mode == emida_amd.MFMA_F16_SPLIT
 
 
 def test_synthetic_grid_matches_benchmark_definition():

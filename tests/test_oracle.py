@@ -414,3 +414,42 @@ def test_get_offset_integer_translation_sign():
     dfm = big[10 + 2:58 + 2, 10 - 3:58 - 3]  # dfm[y][x] = ref[y+2][x-3]
     offs, _ = O.get_offset(dfm.ravel().copy(), ref.ravel().copy(), (48, 48), [(8, 8)], (32, 32), (63, 63), 1, 3)
     assert np.allclose(offs[0], [-3, 2], atol=0.05)
+
+
+# ------------------------------------------------------------------ conditioning of a result
+def test_peak_stats_equal_numpy_of_the_correlation():
+    # get_offset's per-slice statistics: largest value, second largest (ties count
+    # twice) and largest magnitude of the correlation it took the arg-max over
+    rng = np.random.default_rng(21)
+    src, sl, cross, nb, batch = (40, 24), (8, 8), (9, 11), 3, 2
+    begins = [(0, 0), (16, 8), (30, 14)]
+    pics = rng.integers(0, 4000, size=src[0] * src[1] * batch).astype(np.uint16)
+    temp = rng.integers(0, 4000, size=src[0] * src[1]).astype(np.uint16)
+    for policy, c in ((O.POLICY_BRUTE, cross), (O.POLICY_FFT, (15, 15))):
+        *_, st = O.get_offset(pics, temp, src, begins, sl, cross, batch, 3, policy, np.float64, return_stats=True)
+        hx = O.hann(8)
+        p = O.prepare(pics, hx, hx, O.sums(pics, begins, src, sl, batch), begins, src, sl, None, batch)
+        r = O.prepare(temp, hx, hx, O.sums(temp, begins, src, sl, 1), begins, src, sl, None, 1)
+        corr = (O.cross_corr(p, r, sl, c, nb, batch) if policy == O.POLICY_BRUTE else
+                O.fft_corr(p, r, sl, nb, batch)).reshape(nb * batch, -1)
+        top = np.sort(corr, axis=1)
+        np.testing.assert_array_equal(st[:, 0], top[:, -1])
+        np.testing.assert_array_equal(st[:, 1], top[:, -2])
+        np.testing.assert_array_equal(st[:, 2], np.abs(corr).max(axis=1))
+
+
+def test_well_conditioned_definition():
+    # f = a + bx + cy + dx^2 + exy + fy^2 with its maximum at (1.3, 0.8) inside a 3 x 3
+    # neighbourhood, then a saddle, a maximum outside, and a peak without margin
+    def coefs(px, py, d=-1.0, f=-2.0, e=0.3):
+        b = -(2 * d * px + e * py)
+        c = -(2 * f * py + e * px)
+        return [0.0, b, c, d, e, f]
+
+    cf = np.array([coefs(1.3, 0.8), coefs(1.3, 0.8, d=1.0), coefs(2.5, 1.0), coefs(1.0, 1.0)])
+    st = np.array([[10.0, 5.0, 10.0], [10.0, 5.0, 10.0], [10.0, 5.0, 10.0], [10.0, 10.0 - 1e-6, 10.0]])
+    np.testing.assert_array_equal(O.well_conditioned(cf, 3, st), [True, False, False, False])
+    # the margin is relative to the largest magnitude
+    st2 = st.copy()
+    st2[3] = [10.0, 10.0 - 1e-2, 10.0]
+    assert O.well_conditioned(cf, 3, st2)[3]

@@ -131,6 +131,8 @@ def lib():
             "emida_run_prepare_pics": ([i32, i32, i32, vp, vp, vp, vp, vp, vp, S, S, S, u32, u32, vp], i32),
             "emida_run_cross_corr": ([i32, vp, vp, vp, S, S, u32, u32, vp], i32),
             "emida_run_cross_corr_fft": ([i32, vp, vp, vp, S, u32, u32, vp], i32),
+            "emida_run_cross_corr_fft_noalloc": ([i32, vp, vp, vp, S, u32, u32, vp], i32),
+            "emida_fft_noalloc_scratch_bytes": ([ctypes.POINTER(sz)], i32),
             "emida_run_cross_corr_mfma": ([i32, vp, vp, vp, S, S, u32, u32, vp], i32),
             "emida_run_hadamard": ([i32, vp, vp, S, u32, u32, vp], i32),
             "emida_run_maxarg_reduce": ([i32, vp, vp, S, u32, vp], i32),

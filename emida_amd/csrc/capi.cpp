@@ -376,6 +376,25 @@ int emida_run_cross_corr_fft(emida_type T, const void* pics, const void* ref, vo
     });
 }
 
+int emida_run_cross_corr_fft_noalloc(emida_type T, const void* pics, const void* ref, void* res, emida_size2 size,
+                                     uint32_t ref_slices, uint32_t batch_size, void* stream) {
+    return guarded([&] {
+        if (!pics || !ref || !res) throw arg_error("null argument");
+        with_T(T, [&](auto t) {
+            using TT = decltype(t);
+            launch_fft_corr_prepared_noalloc<TT>((const TT*)pics, (const TT*)ref, (TT*)res, S2(size), ref_slices,
+                                                 batch_size, (hipStream_t)stream);
+        });
+    });
+}
+
+int emida_fft_noalloc_scratch_bytes(size_t* bytes) {
+    return guarded([&] {
+        if (!bytes) throw arg_error("null argument");
+        *bytes = kFftStaticScratchBytes;
+    });
+}
+
 int emida_run_cross_corr_mfma(emida_mfma_mode mode, const float* pics, const float* ref, float* res,
                               emida_size2 size, emida_size2 res_size, uint32_t ref_slices, uint32_t batch_size,
                               void* stream) {

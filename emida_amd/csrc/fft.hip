@@ -42,6 +42,9 @@
 //     transform yields both real correlation rows;
 //  D. arg-max over the (2s)^2 circular buffer (reference layout), F x F gather through the FFT
 //     position policy, double-precision least-squares fit, finalize.
+#include <mutex>
+#include <vector>
+
 #include "fft_common.hpp"
 
 namespace emida_amd {
@@ -521,8 +524,74 @@ void launch_fft_corr_prepared(const T* pics, const T* ref, T* res, size2 side, u
     EMIDA_HIP(hipFreeAsync(spectra, st));
 }
 
+// ---- allocation-free variant: working memory in a static device array
+// Each GPU holds its own copy of `scratch` for the life of the process; host code
+// uses the address hipGetSymbolAddress gives for the current device, never the
+// array's name.  Nothing in it needs clearing: every spectrum and work-buffer
+// element is written before it is read.
+static __device__ __align__(256) unsigned char scratch[kFftStaticScratchBytes];
+
+static void* static_scratch() {
+    static std::mutex mu;
+    static std::vector<void*> per_device;
+    int dev = 0;
+    EMIDA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((size_t)dev >= per_device.size()) per_device.resize(dev + 1, nullptr);
+    if (!per_device[dev]) {
+        void* p = nullptr;
+        EMIDA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw hip_error("hipGetSymbolAddress returned no address for the FFT scratch array");
+        per_device[dev] = p;
+    }
+    return per_device[dev];
+}
+
+template <class T>
+void launch_fft_corr_prepared_noalloc(const T* pics, const T* ref, T* res, size2 side, uint32_t ref_slices,
+                                      uint32_t batch, hipStream_t st) {
+    if (ref_slices == 0 || batch == 0) return;
+    if (side.x < 2 || side.y < 2 || side.x > kFftMaxSlice || side.y > kFftMaxSlice)
+        throw unsupported_error("cross_corr_fft: slice sides must be between 2 and " + std::to_string(kFftMaxSlice));
+    const bool dbl = sizeof(T) == 8, general = fftg_needed(dbl, side);
+    const size_t spec_one = general ? fftg_spectrum_bytes(dbl, side) : spectra_bytes<T>(side.x, 1);
+    const size_t spec = (spec_one * ref_slices + 255) & ~(size_t)255;
+    if (spec > kFftStaticScratchBytes)
+        throw arg_error("cross_corr_fft without allocation: the reference spectra of " + std::to_string(ref_slices) +
+                        " slices need " + std::to_string(spec) + " bytes, more than the " +
+                        std::to_string(kFftStaticScratchBytes) + "-byte static scratch");
+    // the rest holds the work buffer of one launch chunk
+    const size_t left = kFftStaticScratchBytes - spec;
+    char* base = nullptr;
+    if (general) {
+        const size_t chunk = std::min<size_t>(left / fftg_slice_work_bytes(dbl, side), (size_t)ref_slices * batch);
+        if (chunk == 0)
+            throw arg_error("cross_corr_fft without allocation: no room for the work buffer beside " +
+                            std::to_string(ref_slices) + " reference spectra in the " +
+                            std::to_string(kFftStaticScratchBytes) + "-byte static scratch");
+        base = (char*)static_scratch();
+        launch_fftg_corr_prepared<T>(pics, ref, res, side, ref_slices, batch, (cplx<T>*)base, base + spec, chunk, st);
+        return;
+    }
+    uint32_t chunk = 0;
+    if (fft_padded(side.x) > 128) {
+        // launch chunks of whole sets of ROIs whose work buffers fit the rest
+        chunk = (uint32_t)std::min<size_t>(left / fft256_slice_work_bytes() / ref_slices * ref_slices,
+                                           (size_t)ref_slices * batch);
+        if (chunk == 0)
+            throw arg_error("cross_corr_fft without allocation: " + std::to_string(ref_slices) + " slices of side " +
+                            std::to_string(side.x) + " need more than the " + std::to_string(kFftStaticScratchBytes) +
+                            "-byte static scratch");
+    }
+    base = (char*)static_scratch();
+    fft_corr_prepared<T>(pics, ref, res, side.x, ref_slices, batch, (cplx<T>*)base, chunk ? base + spec : nullptr,
+                         chunk, st);
+}
+
 #define INST_CORR_PREPARED(T)                                                                                   \
-    template void launch_fft_corr_prepared<T>(const T*, const T*, T*, size2, uint32_t, uint32_t, hipStream_t);
+    template void launch_fft_corr_prepared<T>(const T*, const T*, T*, size2, uint32_t, uint32_t, hipStream_t);   \
+    template void launch_fft_corr_prepared_noalloc<T>(const T*, const T*, T*, size2, uint32_t, uint32_t,         \
+                                                      hipStream_t);
 INST_CORR_PREPARED(float)
 INST_CORR_PREPARED(double)
 

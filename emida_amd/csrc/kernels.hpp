@@ -102,6 +102,14 @@ void launch_fft_pipe(const IN* pics, const S* sums, const T* hx, const T* hy, co
 template <class T>
 void launch_fft_corr_prepared(const T* pics, const T* ref, T* res, size2 side, uint32_t ref_slices, uint32_t batch,
                               hipStream_t st);
+// The same without any allocation: the reference spectra (and the work buffer of the
+// 256-point path) live in the library's static device array of kFftStaticScratchBytes
+// per GPU (shared by every call on that GPU: overlapping calls race).  Throws
+// arg_error when the spectra (plus one work chunk of ref_slices slices) exceed it.
+constexpr size_t kFftStaticScratchBytes = (size_t)32 << 20;
+template <class T>
+void launch_fft_corr_prepared_noalloc(const T* pics, const T* ref, T* res, size2 side, uint32_t ref_slices,
+                                      uint32_t batch, hipStream_t st);
 template <class T>
 void launch_hadamard(cplx<T>* pics, const cplx<T>* ref, size2 one, uint32_t ref_slices, uint32_t batch,
                      hipStream_t st);

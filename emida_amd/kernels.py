@@ -93,6 +93,21 @@ def run_cross_corr_fft(pics, ref, size, ref_slices=1, batch_size=1):
     return out.download(pics.dtype)
 
 
+def run_cross_corr_fft_device(d_pics, d_ref, d_res, dtype, size, ref_slices=1, batch_size=1, noalloc=False,
+                              stream=None):
+    """cross_corr_fft on device buffers already in place (no upload or download);
+    noalloc: the allocation-free entry point, working in the library's static scratch."""
+    fn = C.lib().emida_run_cross_corr_fft_noalloc if noalloc else C.lib().emida_run_cross_corr_fft
+    C.check(fn(C.type_code(np.dtype(dtype)), d_pics.ptr, d_ref.ptr, d_res.ptr, C.size2(size), ref_slices, batch_size,
+               None if stream is None else ctypes.c_void_p(int(stream))))
+
+
+def fft_noalloc_scratch_bytes():
+    v = ctypes.c_size_t()
+    C.check(C.lib().emida_fft_noalloc_scratch_bytes(ctypes.byref(v)))
+    return v.value
+
+
 def run_hadamard(pics, ref, one_size, ref_slices=1, batch_size=1):
     """run_hadamard (src/hadamard.cu:43-49): complex arrays, pics * conj(ref)."""
     pics = np.ascontiguousarray(pics)

@@ -248,6 +248,22 @@ int emida_run_cross_corr(emida_type T, const void* pics, const void* ref, void* 
 int emida_run_cross_corr_fft(emida_type T, const void* pics, const void* ref, void* res, emida_size2 size,
                              uint32_t ref_slices, uint32_t batch_size, void* stream);
 
+/* emida_run_cross_corr_fft without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as emida_run_cross_corr_fft (bit for bit); the reference spectra, and for
+ * slice sides above 64 (float) the column-stage work buffer, live in a static device
+ * array of the library, emida_fft_noalloc_scratch_bytes() = 32 MiB, one per GPU, held
+ * until the process ends.  EMIDA_ERR_ARG when ref_slices spectra (66.5 KiB per slice
+ * of side 64 in float, twice that in double; plus 264 KiB of work per slice for sides
+ * 65..128) do not fit.  Every call on a GPU shares that GPU's array, and each call
+ * leaves its contents there: calls that overlap on one GPU (other streams, other host
+ * threads) race and must be serialised by the caller. */
+int emida_run_cross_corr_fft_noalloc(emida_type T, const void* pics, const void* ref, void* res, emida_size2 size,
+                                     uint32_t ref_slices, uint32_t batch_size, void* stream);
+/* Size of the static scratch array emida_run_cross_corr_fft_noalloc works in. */
+int emida_fft_noalloc_scratch_bytes(size_t* bytes);
+
 /* Brute-force correlation of prepared float slices on matrix cores (the computation
  * of run_cross_corr, src/cross_corr.cu:68-74, in the given operand precision, FP32
  * accumulation, float results).  No reference counterpart.  size.x a multiple of 8

@@ -12,13 +12,21 @@ Also reported:
   roofline      the FFT pipeline kernel's algorithmic FLOP rate (HIP events on its
                 stream) against the FP32 peak; HBM traffic per launch from the
                 committed rocprofv3 counter summary when present;
-  cpu_baseline  the CPU oracle (direct correlation, FP64, OpenMP) on a bounded
-                sample of the same workload on this host;
-  error         max |GPU - FP64 oracle| offset over the sample's ROIs whose FP64
-                fit has its maximum inside the neighbourhood (the others are
-                extrapolations of noise; their raw maximum is reported too);
-  latency       p50/p99 time for a pattern's batch to complete (all patterns of a
-                batch complete together).
+  cpu_baseline  the CPU oracle (direct correlation, FP64, OpenMP on the cores this
+                process is granted, and on one core) on a bounded sample of the same
+                workload on this host, with the host's CPU model, nproc and SMT, and
+                a CPU FFT form (scipy pocketfft) for FFT workloads;
+  accuracy      max |GPU - FP64 oracle| offset over the sample's well-conditioned
+                ROIs (FP64 peak margin above float rounding, fitted maximum inside
+                the neighbourhood: the parity tests' criterion), the fraction of ROIs
+                that covers, the integer-peak agreement and the all-ROI maximum;
+  served        (default workload) the same plan behind the request server: per-pattern
+                p50/p99 latency at 2000 requests/s offered, saturation patterns/s, and
+                the plain FP64 brute form served the same way;
+  precision_form (default workload) the precision form's shape (100 ROIs of 128 x 128,
+                fitsize 7, batch 128): refined fp16-split matrix-core brute against the
+                plain FP64 brute form;
+  batch latency p50/p99 time for one batch to complete when submitted alone.
 """
 import argparse
 import json
@@ -67,33 +75,164 @@ def read_traffic(workload, mode=None):
     return float(m.group(1)) if m else None
 
 
+def cpu_threads():
+    """Every core this process may run on, capped by OMP_NUM_THREADS when the host sets
+    it (a GPU host grants each GPU its share of the cores that way)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def host_info():
+    """nproc, the cores this process may run on, the CPU model and its SMT state."""
+    info = {"nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0)), "cpu_model": None,
+            "smt_active": None, "threads_per_core": None}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    info["cpu_model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            info["smt_active"] = f.read().strip() == "1"
+    except OSError:
+        pass
+    try:
+        with open("/sys/devices/system/cpu/cpu0/topology/thread_siblings_list") as f:
+            n = 0
+            for part in f.read().strip().split(","):
+                a, _, b = part.partition("-")
+                n += int(b) - int(a) + 1 if b else 1
+            info["threads_per_core"] = n
+    except (OSError, ValueError):
+        pass
+    return info
+
+
 def cpu_baseline(cfg, ps, begins, slice_size, cross, seconds, threads):
+    """The CPU oracle (the reference's CPU path restated: sums, prepare, direct
+    correlation, arg-max, gather, subpixel fit; FP64, OpenMP over ROIs) on whole
+    patterns of the workload until `seconds` are used.  Returns the patterns done, the
+    time, and the oracle's offsets, coefficients, arg-max indices and peak statistics."""
     from oracle import oracle as O
 
     O.set_threads(threads)
     policy = O.POLICY_FFT if cfg["policy"] == "fft" else O.POLICY_BRUTE
-    done, t_used, offs, coefs = 0, 0.0, [], []
+    done, t_used, offs, coefs, idx, stats = 0, 0.0, [], [], [], []
     while done < ps.pics.shape[0] and (done == 0 or t_used < seconds):
         t0 = time.perf_counter()
-        o, c = O.get_offset(ps.pics[done], ps.ref, ps.size, begins, slice_size, cross, 1, cfg["fit"], policy,
-                            np.float64)
+        o, c, m, _, st = O.get_offset(ps.pics[done], ps.ref, ps.size, begins, slice_size, cross, 1, cfg["fit"], policy,
+                                      np.float64, return_intermediates=True, return_stats=True)
         t_used += time.perf_counter() - t0
         offs.append(o)
         coefs.append(c)
+        idx.append(m["index"])
+        stats.append(st)
         done += 1
-    return done, t_used, np.concatenate(offs), np.concatenate(coefs)
+    return done, t_used, np.concatenate(offs), np.concatenate(coefs), np.concatenate(idx), np.concatenate(stats)
 
 
-def well_posed(coefs, fit):
-    """ROIs whose FP64 quadratic fit has a maximum inside the F x F neighbourhood.
-    Elsewhere (noise-dominated ROIs) the fitted peak is an extrapolation whose
-    position is ill-conditioned in any precision."""
-    b, c, d, e, f = (coefs[:, i] for i in range(1, 6))
-    det = 4 * f * d - e * e
-    with np.errstate(divide="ignore", invalid="ignore"):
-        py = (b * e - 2 * c * d) / det
-        px = (-e * py - b) / (2 * d)
-    return (d < 0) & (det > 0) & (px >= 0) & (px <= fit - 1) & (py >= 0) & (py <= fit - 1)
+def cpu_single_thread(cfg, ps, begins, slice_size, cross, seconds):
+    """The same CPU oracle on one thread, over ROIs of the first pattern in groups of 20
+    until `seconds` are used; patterns/s scaled from the ROIs done."""
+    from oracle import oracle as O
+
+    O.set_threads(1)
+    policy = O.POLICY_FFT if cfg["policy"] == "fft" else O.POLICY_BRUTE
+    done, t_used = 0, 0.0
+    while done < len(begins) and (done == 0 or t_used < seconds):
+        sub = begins[done:done + 20]
+        t0 = time.perf_counter()
+        O.get_offset(ps.pics[0], ps.ref, ps.size, sub, slice_size, cross, 1, cfg["fit"], policy, np.float64)
+        t_used += time.perf_counter() - t0
+        done += len(sub)
+    return done / len(begins) / t_used, done, t_used
+
+
+def cpu_fft_baseline(cfg, ps, begins, slice_size, seconds, threads):
+    """A CPU FFT form of the pipeline (the reference's CPU path has no FFT): the
+    oracle's sums and preparation, scipy's pocketfft R2C of every 2s x 2s zero-padded
+    slice (reference spectra computed once), product with the conjugate spectrum,
+    C2R, then the oracle's arg-max, gather (FFT position policy) and fit; FP64,
+    `threads` FFT workers.  Returns patterns/s and the patterns done."""
+    import scipy.fft as sf
+    from oracle import oracle as O
+
+    O.set_threads(threads)
+    sx, sy = slice_size
+    mx, my = 2 * sx, 2 * sy
+    nb, fit = len(begins), cfg["fit"]
+    hx, hy = O.hann(sx), O.hann(sy)
+    sr = O.sums(ps.ref.ravel(), begins, ps.size, slice_size, 1)
+    r = O.prepare(ps.ref.ravel(), hx, hy, sr, begins, ps.size, slice_size, None, 1).reshape(nb, sy, sx)
+    R = np.conj(sf.rfft2(r, s=(my, mx), workers=threads))
+    done, t_used = 0, 0.0
+    while done < ps.pics.shape[0] and (done == 0 or t_used < seconds):
+        t0 = time.perf_counter()
+        pic = ps.pics[done].ravel()
+        sp = O.sums(pic, begins, ps.size, slice_size, 1)
+        p = O.prepare(pic, hx, hy, sp, begins, ps.size, slice_size, None, 1).reshape(nb, sy, sx)
+        c = sf.irfft2(sf.rfft2(p, s=(my, mx), workers=threads) * R, s=(my, mx), workers=threads)
+        c = np.ascontiguousarray((c * (mx * my)).reshape(nb, -1))  # unnormalised, as a C2R
+        mxi = O.maxarg(c, mx * my, nb)
+        nbh = O.extract(c, mxi, fit, (mx - 1, my - 1), nb, O.POLICY_FFT)
+        O.subpixel(nbh, fit)
+        t_used += time.perf_counter() - t0
+        done += 1
+    return done / t_used, done
+
+
+ACCURACY_DEFINITION = ("max |GPU - FP64 oracle| offset (px) over the well-conditioned ROIs: the FP64 "
+                       "correlation peak beats the runner-up by more than 1e-4 of the largest magnitude and "
+                       "the FP64 quadratic fit has its maximum inside the F x F neighbourhood "
+                       "(oracle.well_conditioned, the parity tests' criterion)")
+
+
+def accuracy(gpu_offs, gpu_index, cpu_offs, cpu_coefs, cpu_index, stats, fit):
+    """The offset error against FP64 under the one definition the parity tests use, the
+    fraction of ROIs it covers, the integer-peak agreement and the all-ROI maximum."""
+    from oracle import oracle as O
+
+    ok = O.well_conditioned(cpu_coefs, fit, stats)
+    err = np.abs(gpu_offs - cpu_offs).max(axis=1)
+    same = gpu_index == cpu_index
+    return {"definition": ACCURACY_DEFINITION, "rois": int(len(err)),
+            "max_err_px": float(err[ok].max()) if ok.any() else None,
+            "mean_err_px": float(err[ok].mean()) if ok.any() else None,
+            "well_conditioned_roi_fraction": round(float(ok.mean()), 4),
+            "integer_peak_match_fraction": round(float(same.mean()), 4),
+            "integer_peak_match_fraction_well_conditioned": round(float(same[ok].mean()), 4) if ok.any() else None,
+            "max_err_px_all_rois": float(np.nanmax(err))}
+
+
+def fp64_fft_peak_stats(pics, ref, src, begins, slice_size, threads):
+    """Per slice the largest value, the runner-up and the largest magnitude of the FP64
+    FFT-policy correlation ((2s)^2 circular buffer) of every pattern in `pics`, from
+    the oracle's preparation and scipy's double FFT - the statistics
+    oracle.well_conditioned reads, at FFT speed for large subsets."""
+    import scipy.fft as sf
+    from oracle import oracle as O
+
+    sx, sy = slice_size
+    nb = len(begins)
+    hx, hy = O.hann(sx), O.hann(sy)
+    sr = O.sums(ref.ravel(), begins, src, slice_size, 1)
+    R = np.conj(sf.rfft2(O.prepare(ref.ravel(), hx, hy, sr, begins, src, slice_size, None, 1).reshape(nb, sy, sx),
+                         s=(2 * sy, 2 * sx), workers=threads))
+    out = []
+    for pic in pics:
+        sp = O.sums(pic.ravel(), begins, src, slice_size, 1)
+        p = O.prepare(pic.ravel(), hx, hy, sp, begins, src, slice_size, None, 1).reshape(nb, sy, sx)
+        c = sf.irfft2(sf.rfft2(p, s=(2 * sy, 2 * sx), workers=threads) * R, s=(2 * sy, 2 * sx),
+                      workers=threads).reshape(nb, -1)
+        top = np.partition(c, -2, axis=1)[:, -2:]
+        out.append(np.stack([top.max(axis=1), top.min(axis=1), np.abs(c).max(axis=1)], axis=1))
+    return np.concatenate(out)
 
 
 def copy_calibration(mb):
@@ -108,6 +247,58 @@ def copy_calibration(mb):
         C.check(C.lib().emida_memcpy_d2d(b.ptr, a.ptr, n))
     emida_amd.synchronize()
     print(json.dumps({"copy_calibration_bytes": n, "copies": 5}))
+
+
+def run_load(srv, pids, pics, n, rate, seed, submitters=1):
+    """n single-pattern requests to the plans `pids` (chosen uniformly) of a request
+    server; Poisson arrivals at `rate` per second from one thread, or back to back from
+    `submitters` threads when rate is None.  Returns per-request latencies (ms), the
+    span from the first submission to the last completion (s), the outputs and the
+    mean batch size."""
+    import queue
+    import threading
+
+    rng = np.random.default_rng(seed)
+    choice = rng.integers(0, len(pids), n)
+    pat = rng.integers(0, pics.shape[0], n)
+    gaps = rng.exponential(1.0 / rate, n) if rate else np.zeros(n)
+    arrival = np.cumsum(gaps)
+    rids, outs, infos = [None] * n, [None] * n, [None] * n
+    done_q = queue.Queue()
+
+    def waiter():
+        for c in range(n):
+            i = done_q.get()
+            infos[i] = srv.wait(rids[i])
+            if c % 2000 == 1999:
+                print(f"# {c + 1}/{n} requests done", file=sys.stderr, flush=True)
+
+    def submitter(idx, t0):
+        for i in idx:
+            if rate:
+                while True:
+                    d = t0 + arrival[i] - time.perf_counter()
+                    if d <= 0:
+                        break
+                    time.sleep(d if d > 2e-4 else 0)
+            rid, offs, _ = srv.submit(pids[choice[i]], pics[pat[i]])
+            rids[i], outs[i] = rid, (choice[i], pat[i], offs)
+            done_q.put(i)
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    subs = [threading.Thread(target=submitter, args=(range(k, n, submitters), t0)) for k in range(submitters)]
+    for t in subs:
+        t.start()
+    for t in subs:
+        t.join()
+    th.join()
+    lat = np.array([inf["latency_ms"] for inf in infos])
+    t_first = min(inf["submit_ns"] for inf in infos)
+    t_last = max(inf["done_ns"] for inf in infos)
+    return dict(lat=lat, span_s=(t_last - t_first) * 1e-9, outs=outs,
+                batch_mean=float(np.mean([inf["batch"] for inf in infos])))
 
 
 def serve_plans(cfg, plain_fp64=False):
@@ -154,82 +345,40 @@ def serve_main(args, cfg):
             plans.append(dict(pid=pid, r=r, k=k, T=T, pol=pol, begins=begins))
         return srv, plans
 
-    def run_load(srv, plans, n, rate, seed, submitters=1):
-        """n requests; Poisson arrivals at `rate` per second from one thread, or back to
-        back from `submitters` threads when rate is None."""
-        import queue
-
-        rng = np.random.default_rng(seed)
-        choice = rng.integers(0, len(plans), n)
-        pat = rng.integers(0, ps.pics.shape[0], n)
-        gaps = rng.exponential(1.0 / rate, n) if rate else np.zeros(n)
-        arrival = np.cumsum(gaps)
-        rids, outs, infos = [None] * n, [None] * n, [None] * n
-        done_q = queue.Queue()
-
-        def waiter():
-            for c in range(n):
-                i = done_q.get()
-                infos[i] = srv.wait(rids[i])
-                if c % 2000 == 1999:
-                    print(f"# {c + 1}/{n} requests done", file=sys.stderr, flush=True)
-
-        def submitter(idx, t0):
-            for i in idx:
-                if rate:
-                    while True:
-                        d = t0 + arrival[i] - time.perf_counter()
-                        if d <= 0:
-                            break
-                        time.sleep(d if d > 2e-4 else 0)
-                rid, offs, _ = srv.submit(plans[choice[i]]["pid"], ps.pics[pat[i]])
-                rids[i], outs[i] = rid, (choice[i], pat[i], offs)
-                done_q.put(i)
-
-        th = threading.Thread(target=waiter)
-        th.start()
-        t0 = time.perf_counter()
-        subs = [threading.Thread(target=submitter, args=(range(k, n, submitters), t0)) for k in range(submitters)]
-        for t in subs:
-            t.start()
-        for t in subs:
-            t.join()
-        th.join()
-        lat = np.array([inf["latency_ms"] for inf in infos])
-        t_first = min(inf["submit_ns"] for inf in infos)
-        t_last = max(inf["done_ns"] for inf in infos)
-        return dict(lat=lat, span_s=(t_last - t_first) * 1e-9, outs=outs,
-                    batch_mean=float(np.mean([inf["batch"] for inf in infos])))
-
     srv, plans = make_server(False)
+    pids = [p["pid"] for p in plans]
     # warm-up: every plan captures its graphs for a few batch sizes
-    run_load(srv, plans, 40 * len(plans), None, 1, submitters=4)
+    run_load(srv, pids, ps.pics, 40 * len(plans), None, 1, submitters=4)
     n_load = int(cfg["rate"] * args.duration)
     print(f"# offered load: {n_load} requests at {cfg['rate']:.0f}/s", file=sys.stderr, flush=True)
-    load = run_load(srv, plans, n_load, cfg["rate"], 2)
+    load = run_load(srv, pids, ps.pics, n_load, cfg["rate"], 2)
     n_sat = args.saturation_requests
-    sat = run_load(srv, plans, n_sat, None, 3, submitters=4)
+    sat = run_load(srv, pids, ps.pics, n_sat, None, 3, submitters=4)
     st = srv.stats()
 
-    # accuracy against the FP64 oracle on one pattern per plan (every 4th ROI)
+    # accuracy against the FP64 oracle (the plan's policy) on one pattern per plan,
+    # every 8th ROI, over the well-conditioned ROIs
     from oracle import oracle as O
-    O.set_threads(min(16, len(os.sched_getaffinity(0))))
+    O.set_threads(cpu_threads())
     errs = {}
     for p in plans:
         rid, offs, coefs = srv.submit(p["pid"], ps.pics[0])
         srv.wait(rid)
-        sub = p["begins"][::4]
-        eo, ec = O.get_offset(ps.pics[0], ps.ref, ps.size, sub, (2 * p["r"], 2 * p["r"]),
-                              (4 * p["r"] - 1, 4 * p["r"] - 1), 1, fit, O.POLICY_BRUTE, np.float64)
-        ok = well_posed(ec, fit)
-        e = np.abs(offs[::4] - eo).max(axis=1)
+        sub = p["begins"][::8]
+        pol = O.POLICY_FFT if p["pol"] == emida_amd.CROSS_POLICY_FFT else O.POLICY_BRUTE
+        eo, ec, _, _, est = O.get_offset(ps.pics[0], ps.ref, ps.size, sub, (2 * p["r"], 2 * p["r"]),
+                                         (4 * p["r"] - 1, 4 * p["r"] - 1), 1, fit, pol, np.float64,
+                                         return_intermediates=True, return_stats=True)
+        ok = O.well_conditioned(ec, fit, est)
+        e = np.abs(offs[::8] - eo).max(axis=1)
         errs[f"r{p['r']}_k{p['k']}"] = float(e[ok].max()) if ok.any() else None
     srv.close()
 
     # the same request mix through the plain FP64 brute-force form (saturation)
     srv64, plans64 = make_server(True)
-    run_load(srv64, plans64, len(plans64), None, 4)
-    sat64 = run_load(srv64, plans64, args.fp64_requests, None, 3, submitters=4)
+    pids64 = [p["pid"] for p in plans64]
+    run_load(srv64, pids64, ps.pics, len(plans64), None, 4)
+    sat64 = run_load(srv64, pids64, ps.pics, args.fp64_requests, None, 3, submitters=4)
     srv64.close()
 
     sat_rate = n_sat / sat["span_s"]
@@ -257,7 +406,7 @@ def serve_main(args, cfg):
                           "mean_batch": round(sat["batch_mean"], 2)},
            "plain_fp64_brute_served_patterns_per_s": round(sat64_rate, 2),
            "served_over_plain_fp64": round(sat_rate / sat64_rate, 2),
-           "max_err_px_vs_fp64_per_plan": errs,
+           "max_err_px_vs_fp64_per_plan": errs, "accuracy_definition": ACCURACY_DEFINITION,
            "server_stats": {k: int(v) for k, v in st.items()}}
     print(json.dumps(out))
     return 0
@@ -266,8 +415,9 @@ def serve_main(args, cfg):
 def scan_main(args, cfg):
     """Scan workload: patterns streamed back to back through the request loop, once
     with the FFT policy in float and once with split-bf16 matrix-core brute
-    correlation; offsets of a subset compared with FP64 brute force on the GPU (the
-    FP64 brute kernels reproduce the CPU oracle bit for bit)."""
+    correlation; offsets of a subset of distinct raster points (cfg["subset"], 500)
+    compared with FP64 brute force on the GPU (the FP64 brute kernels reproduce the
+    CPU oracle bit for bit) over the well-conditioned ROIs."""
     import queue
     import threading
 
@@ -276,10 +426,18 @@ def scan_main(args, cfg):
 
     t_gen = time.perf_counter()
     sc = synth.ScanSet(args.workload, cfg["size"], args.unique, t_range=cfg["t_range"])
+    # the error subset: distinct raster points of their own (rendered before the GPU is
+    # touched, in a process pool), apart from the points the timed stream cycles over
+    sub = cfg["subset"]
+    if sub > args.unique:
+        more, _ = sc.render(range(args.unique, sub), workers=cpu_threads())
+        sub_pics = np.concatenate([sc.pics, more])
+    else:
+        sub_pics = sc.pics[:sub]
     t_gen = time.perf_counter() - t_gen
     dev_name, cus = emida_amd.device_query()
-    print(f"# {dev_name}, {cus} CUs; rendered {args.unique} raster points in {t_gen:.1f}s", file=sys.stderr,
-          flush=True)
+    print(f"# {dev_name}, {cus} CUs; rendered {max(sub, args.unique)} raster points in {t_gen:.1f}s",
+          file=sys.stderr, flush=True)
     r, fit = cfg["radius"], cfg["fit"]
     s2, cross = (2 * r, 2 * r), (4 * r - 1, 4 * r - 1)
     begins = synth.grid_begins(cfg["size"], cfg["k"], r)
@@ -319,16 +477,22 @@ def scan_main(args, cfg):
         th.join()
         span = (max(i["done_ns"] for i in infos) - min(i["submit_ns"] for i in infos)) * 1e-9
         lat = np.array([i["latency_ms"] for i in infos])
+        # the error subset through the same server (not timed)
+        reqs = [srv.submit(pid, sub_pics[i]) for i in range(sub)]
+        sub_out = []
+        for rid, o, _ in reqs:
+            srv.wait(rid)
+            sub_out.append(o.copy())
         srv.close()
-        return n / span, lat, outs
+        return n / span, lat, sub_out
 
     fft_rate, fft_lat, fft_out = stream(emida_amd.CROSS_POLICY_FFT, 0)
     # matrix-core screen with the exact peak in double: the FP64 brute results
     mfma_rate, mfma_lat, mfma_out = stream(emida_amd.CROSS_POLICY_BRUTE_MFMA, emida_amd.MFMA_BF16_SPLIT, np.float64)
     f16_rate, f16_lat, f16_out = stream(emida_amd.CROSS_POLICY_BRUTE_MFMA, emida_amd.MFMA_F16_SPLIT, np.float64)
 
-    # FP64 reference offsets of the rendered raster points (the subset), brute force on the GPU
-    sub = min(cfg["subset"], args.unique)
+    # FP64 reference offsets of the subset, brute force on the GPU, and the FP64
+    # correlation's peak statistics (the well-conditioned criterion)
     with emida_amd.GpuOffset(np.float64, np.uint16, sc.size, begins, s2, cross, 16, emida_amd.CROSS_POLICY_BRUTE,
                              fit) as g:
         g.allocate_memory(sc.ref)
@@ -336,13 +500,15 @@ def scan_main(args, cfg):
         for b0 in range(0, sub, 16):
             pics = np.zeros((16,) + sc.pics.shape[1:], np.uint16)
             m = min(16, sub - b0)
-            pics[:m] = sc.pics[b0:b0 + m]
+            pics[:m] = sub_pics[b0:b0 + m]
             o, c = g.get_offset(pics)
             ref_offs.append(o.reshape(16, -1, 2)[:m])
             ref_coefs.append(c.reshape(16, -1, 6)[:m])
     ref_offs = np.concatenate(ref_offs)
     ref_coefs = np.concatenate(ref_coefs)
-    ok = well_posed(ref_coefs.reshape(-1, 6), fit).reshape(sub, -1)
+    from oracle import oracle as O
+    stats = fp64_fft_peak_stats(sub_pics, sc.ref, sc.size, begins, s2, cpu_threads())
+    ok = O.well_conditioned(ref_coefs.reshape(-1, 6), fit, stats).reshape(sub, -1)
 
     def err(outs):
         e = np.stack([np.abs(outs[i] - ref_offs[i]).max(axis=1) for i in range(sub)])
@@ -369,7 +535,8 @@ def scan_main(args, cfg):
                               "roi_offsets_per_s": round(f16_rate * len(begins)),
                               "p50_pattern_latency_ms": round(float(np.percentile(f16_lat, 50)), 3),
                               "err_vs_fp64": err(f16_out)},
-           "well_posed_roi_fraction": round(float(ok.mean()), 4),
+           "accuracy_definition": ACCURACY_DEFINITION, "fp64_subset_patterns": sub,
+           "well_conditioned_roi_fraction": round(float(ok.mean()), 4),
            "pcie_floor_patterns_per_s": round(50e9 / (cfg["size"] ** 2 * 2), 1)}
     print(json.dumps(out))
     return 0
@@ -555,6 +722,13 @@ def main():
     ap.add_argument("--saturation-requests", type=int, default=6000, help="serving: back-to-back requests")
     ap.add_argument("--fp64-requests", type=int, default=90, help="serving: requests through the FP64 form")
     ap.add_argument("--scan-patterns", type=int, default=0, help="scan: patterns streamed (0: the workload's)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default workload: skip the served-form and precision-form blocks")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="served block: offered requests per second")
+    ap.add_argument("--serve-seconds", type=float, default=3.0, help="served block: seconds of offered load")
+    ap.add_argument("--serve-saturation", type=int, default=3000, help="served block: back-to-back requests")
+    ap.add_argument("--serve-fp64-requests", type=int, default=256, help="served block: requests to the FP64 form")
+    ap.add_argument("--precision-steps", type=int, default=6, help="precision block: timed batches of 128")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -699,27 +873,132 @@ def main():
                       "fitsize": cfg["fit"], "batch": batch, "batches_in_flight": len(gs),
                       "distinct_patterns": args.unique, "device": dev_name},
            "roi_offsets_per_s": round(pats_per_s * n_roi, 1),
-           "p50_pattern_latency_ms": round(float(np.percentile(lat, 50)), 4),
-           "p99_pattern_latency_ms": round(float(np.percentile(lat, 99)), 4),
+           # one batch of `batch` patterns submitted alone and waited for
+           "batch_latency_p50_ms": round(float(np.percentile(lat, 50)), 4),
+           "batch_latency_p99_ms": round(float(np.percentile(lat, 99)), 4),
            "roofline": roofline}
 
+    gpu_index = d_maxes.download(g.data_index, total)["index"]
+    for gi in gs:
+        gi.close()
+    del d_pics
+
+    extras = args.workload == "fft_f32_873" and not args.no_extras
+    if extras:
+        # the serving form and the precision form at their own shapes, on this GPU
+        out["served"] = served_block(args, cfg, ps, begins, slice_size, cross)
+        # per-pattern latency: single-pattern requests to the request server at the offered load
+        out["p50_pattern_latency_ms"] = out["served"]["p50_pattern_latency_ms"]
+        out["p99_pattern_latency_ms"] = out["served"]["p99_pattern_latency_ms"]
+        out["precision_form"] = precision_block(args, ps)
+
     if not args.no_cpu_baseline:
-        threads = min(16, len(os.sched_getaffinity(0)))
-        done, t_used, cpu_offs, cpu_coefs = cpu_baseline(cfg, ps, begins, slice_size, cross, args.cpu_seconds,
-                                                         threads)
+        threads = cpu_threads()
+        done, t_used, cpu_offs, cpu_coefs, cpu_index, stats = cpu_baseline(cfg, ps, begins, slice_size, cross,
+                                                                           args.cpu_seconds, threads)
         cpu_rate = done / t_used
-        err = np.abs(gpu_offs[:done * n_roi] - cpu_offs).max(axis=1)
-        ok = well_posed(cpu_coefs, cfg["fit"])
+        n = done * n_roi
+        acc = accuracy(gpu_offs[:n], gpu_index[:n], cpu_offs, cpu_coefs, cpu_index, stats, cfg["fit"])
+        one_rate, one_rois, one_t = cpu_single_thread(cfg, ps, begins, slice_size, cross, args.cpu_seconds / 3)
         out["cpu_baseline"] = {"value": round(cpu_rate, 4), "unit": "patterns/s", "cores": threads, "kind": "port",
                                "sample": f"{done} pattern(s) x {n_roi} ROIs of the same workload through the CPU "
-                                         f"oracle (direct correlation, FP64, OpenMP), {t_used:.1f}s"}
-        out["max_err_px_vs_fp64"] = float(err[ok].max())
-        out["well_posed_roi_fraction"] = round(float(ok.mean()), 4)
-        out["max_err_px_vs_fp64_all_rois"] = float(np.nanmax(err))
+                                         f"oracle (the reference's CPU path restated: direct correlation, FP64, "
+                                         f"OpenMP over ROIs, -O3 -mavx2 -mfma), {t_used:.1f}s",
+                               "single_thread": {"value": round(one_rate, 4), "unit": "patterns/s", "cores": 1,
+                                                 "sample": f"{one_rois} ROIs of one pattern, {one_t:.1f}s"},
+                               "host": host_info()}
+        if cfg["policy"] == "fft":
+            fft_rate, fft_done = cpu_fft_baseline(cfg, ps, begins, slice_size, args.cpu_seconds / 3, threads)
+            out["cpu_baseline"]["cpu_fft"] = {
+                "value": round(fft_rate, 3), "unit": "patterns/s", "cores": threads,
+                "label": "CPU FFT form (no counterpart in the reference's CPU path): oracle sums/prepare, scipy "
+                         "pocketfft R2C / product / C2R in FP64, oracle arg-max, gather and fit",
+                "sample": f"{fft_done} pattern(s) x {n_roi} ROIs"}
+            out["gpu_over_cpu_fft"] = round(pats_per_s / fft_rate, 1)
+        out["accuracy"] = acc
+        out["max_err_px_vs_fp64"] = acc["max_err_px"]
+        out["well_conditioned_roi_fraction"] = acc["well_conditioned_roi_fraction"]
+        out["integer_peak_match_fraction"] = acc["integer_peak_match_fraction"]
+        out["max_err_px_vs_fp64_all_rois"] = acc["max_err_px_all_rois"]
         out["gpu_over_cpu"] = round(pats_per_s / cpu_rate, 1)
+        out["gpu_over_cpu_single_thread"] = round(pats_per_s / one_rate, 1)
     print(json.dumps(out))
-    g.close()
     return 0
+
+
+def served_block(args, cfg, ps, begins, slice_size, cross):
+    """The serving form at this workload's shape: single-pattern requests to a request
+    server holding the plan resident - Poisson arrivals at args.serve_rate for
+    args.serve_seconds (per-pattern p50/p99 latency), then back-to-back requests
+    (saturation patterns/s) - and the same requests served by the plain FP64 brute form."""
+    import emida_amd
+
+    def server(T, policy):
+        srv = emida_amd.Server(hbm_budget=16 << 30, max_batch=args.serve_batch, streams=4, max_queued=4096,
+                               slots_per_plan=2)
+        pid = srv.add_plan(T, ps.ref, begins, slice_size, cross, policy, cfg["fit"])
+        return srv, [pid]
+
+    srv, pids = server(np.float32, emida_amd.CROSS_POLICY_FFT)
+    with srv:
+        run_load(srv, pids, ps.pics, 256, None, 1, submitters=4)  # graphs of every batch size
+        n_load = max(1, int(args.serve_rate * args.serve_seconds))
+        load = run_load(srv, pids, ps.pics, n_load, args.serve_rate, 2)
+        sat = run_load(srv, pids, ps.pics, args.serve_saturation, None, 3, submitters=4)
+    srv64, pids64 = server(np.float64, emida_amd.CROSS_POLICY_BRUTE)
+    with srv64:
+        run_load(srv64, pids64, ps.pics, 8, None, 4, submitters=2)
+        sat64 = run_load(srv64, pids64, ps.pics, args.serve_fp64_requests, None, 5, submitters=4)
+    sat_rate = args.serve_saturation / sat["span_s"]
+    sat64_rate = args.serve_fp64_requests / sat64["span_s"]
+    return {"plan": f"{len(begins)} ROIs of {slice_size[0]}x{slice_size[1]}, FFT float, fitsize {cfg['fit']}",
+            "offered_rate": args.serve_rate, "offered_requests": n_load,
+            "achieved_patterns_per_s": round(n_load / load["span_s"], 1),
+            "p50_pattern_latency_ms": round(float(np.percentile(load["lat"], 50)), 4),
+            "p99_pattern_latency_ms": round(float(np.percentile(load["lat"], 99)), 4),
+            "mean_batch": round(load["batch_mean"], 2),
+            "saturation_patterns_per_s": round(sat_rate, 1), "saturation_requests": args.serve_saturation,
+            "plain_fp64_brute_served_patterns_per_s": round(sat64_rate, 2),
+            "plain_fp64_requests": args.serve_fp64_requests,
+            "served_over_plain_fp64": round(sat_rate / sat64_rate, 2)}
+
+
+def precision_block(args, ps):
+    """The precision form at its own shape (873 x 873, 100 ROIs of 128 x 128, the full
+    255 x 255 window, fitsize 7, batch 128): fp16-split matrix-core screen with the exact
+    double peak, against the plain FP64 brute form on the same resident patterns; both
+    give the same results (checked here on one batch)."""
+    import emida_amd
+    from emida_amd import synth
+
+    cfg3 = synth.CONFIGS["brute_873_r64"]
+    r, batch, fit = cfg3["radius"], cfg3["batch"], cfg3["fit"]
+    begins = synth.grid_begins(cfg3["size"], cfg3["k"], r)
+    s2, cross = (2 * r, 2 * r), (4 * r - 1, 4 * r - 1)
+    pics = np.concatenate([ps.pics] * -(-batch // ps.pics.shape[0]))[:batch]
+    d_pics = emida_amd.DeviceBuffer.from_array(pics)
+    total = len(begins) * batch
+    res = {}
+    for name, policy, steps in (("refined_f16_split", emida_amd.CROSS_POLICY_BRUTE_MFMA, args.precision_steps),
+                                ("plain_fp64_brute", emida_amd.CROSS_POLICY_BRUTE, max(1, args.precision_steps // 3))):
+        with emida_amd.GpuOffset(np.float64, np.uint16, ps.size, begins, s2, cross, batch, policy, fit) as g:
+            g.allocate_memory(ps.ref)
+            d_offs = emida_amd.DeviceBuffer(total * 16)
+            g.process_device(d_pics, batch, d_offsets=d_offs)  # warm-up
+            emida_amd.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                g.process_device(d_pics, batch, d_offsets=d_offs)
+            emida_amd.synchronize()
+            el = time.perf_counter() - t0
+            res[name] = (steps * batch / el, d_offs.download(np.float64, 2 * total))
+    same = bool(np.array_equal(res["refined_f16_split"][1], res["plain_fp64_brute"][1], equal_nan=True))
+    return {"shape": f"{cfg3['size']}x{cfg3['size']}, {len(begins)} ROIs of {2 * r}x{2 * r}, window "
+                     f"{cross[0]}x{cross[1]}, fitsize {fit}, batch {batch}",
+            "refined_f16_split_patterns_per_s": round(res["refined_f16_split"][0], 1),
+            "plain_fp64_brute_patterns_per_s": round(res["plain_fp64_brute"][0], 2),
+            "refined_over_plain_fp64": round(res["refined_f16_split"][0] / res["plain_fp64_brute"][0], 2),
+            "results_equal_fp64_brute": same}
 
 
 if __name__ == "__main__":

@@ -150,27 +150,62 @@ def smooth_strain_field(shape, amplitude=2e-3, corr_length=10.0, seed=0):
     return np.stack(comps, axis=-1).reshape(-1, 2, 2)
 
 
+_RENDER = {}
+
+
+def _render_init(clean, eps_field, name, seed, t_range):
+    _RENDER.update(clean=clean, eps_field=eps_field, name=name, seed=seed, t_range=t_range)
+
+
+def _render_point(i):
+    """One raster point of a scan (process-pool worker): its deformed pattern and t."""
+    r = _RENDER
+    rng = np.random.default_rng(seed_of(r["name"], r["seed"], "t", i))
+    t = rng.uniform(-r["t_range"], r["t_range"], 2)
+    pic = poisson_u16(deform(r["clean"], r["eps_field"][i], t), seed_of(r["name"], r["seed"], "def-noise", i))
+    return pic, t
+
+
 class ScanSet:
     """A scan: one reference and deformed patterns at raster points whose strain varies
     smoothly over the raster.  Only the first `unique` raster points are rendered;
-    requests cycle over them."""
+    requests cycle over them.  render() draws any other raster points the same way."""
 
     def __init__(self, name, size, unique, t_range=2.0, n_bands=24, seed=0):
         self.name = name
+        self.seed = seed
+        self.t_range = t_range
         self.size = (size, size) if np.isscalar(size) else tuple(size)
         self.points, shape = scan_raster()
-        eps_field = smooth_strain_field(shape, seed=seed_of(name, seed, "eps"))
+        self.eps_field = smooth_strain_field(shape, seed=seed_of(name, seed, "eps"))
         self.clean = band_reference(self.size, n_bands, seed_of(name, seed, "ref"))
         self.ref = poisson_u16(self.clean, seed_of(name, seed, "ref-noise"))
+        self.pics, self.t = self.render(range(unique))
+        self.eps = [self.eps_field[i] for i in range(unique)]
+
+    def render(self, indices, workers=1):
+        """Deformed patterns (len, h, w) and translations of the raster points `indices`;
+        workers > 1 renders them in a process pool."""
+        indices = list(indices)
         w, h = self.size
-        self.pics = np.empty((unique, h, w), np.uint16)
-        self.eps, self.t = [], []
-        for i in range(unique):
-            rng = np.random.default_rng(seed_of(name, seed, "t", i))
-            t = rng.uniform(-t_range, t_range, 2)
-            self.pics[i] = poisson_u16(deform(self.clean, eps_field[i], t), seed_of(name, seed, "def-noise", i))
-            self.eps.append(eps_field[i])
-            self.t.append(t)
+        pics = np.empty((len(indices), h, w), np.uint16)
+        ts = []
+        args = (self.clean, self.eps_field, self.name, self.seed, self.t_range)
+        if workers > 1 and len(indices) > 1:
+            import concurrent.futures as cf
+            import multiprocessing as mp
+
+            with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("fork"),
+                                        initializer=_render_init, initargs=args) as ex:
+                for k, (pic, t) in enumerate(ex.map(_render_point, indices, chunksize=4)):
+                    pics[k] = pic
+                    ts.append(t)
+        else:
+            _render_init(*args)
+            for k, i in enumerate(indices):
+                pics[k], t = _render_point(i)
+                ts.append(t)
+        return pics, ts
 
 
 # The five workloads of the benchmark configurations (pattern side, patterns,

@@ -237,8 +237,7 @@ int emida_offset_get_timing(emida_offset* h, const char** kernel_name, double* t
         if (kernel_name)
             *kernel_name = h->policy == EMIDA_CROSS_POLICY_FFT
                                ? (h->fftg ? "fftg_rows_kernel+fftg_cols_kernel+fftg_rows_inv_kernel+peak_kernel"
-                                          : fft_padded(h->side) > 128 ? "fft256_cols_kernel+fft256_rows_kernel"
-                                                                      : "fft_kernel")
+                                          : fft_padded(h->side) > 128 ? "fft256_kernel" : "fft_kernel")
                                : (h->policy == EMIDA_CROSS_POLICY_BRUTE_MFMA
                                       ? "mfma_corr_kernel"
                                       : cross_corr_kernel_name(h->T == EMIDA_F64, h->slice, h->cross, h->nb, h->batch));

@@ -49,6 +49,9 @@
 
 namespace emida_amd {
 
+// Bound of the kernel's static LDS (reductions, neighbourhood, least-squares rows).
+constexpr size_t kStaticLds = 5120;
+
 // Geometry of one transform size.
 template <class T, int M>
 struct fft_plan {
@@ -69,10 +72,35 @@ struct fft_plan {
     static constexpr size_t lds_bytes = (size_t)(NC * CS + M) * sizeof(cplx<T>);
     // waves per SIMD the LDS allows (160 KiB per CU, 4 SIMDs); registers are
     // budgeted for exactly that many
-    static constexpr int WG_PER_CU = (int)((160 * 1024) / (lds_bytes + 2048));
+    static constexpr int WG_PER_CU = (int)((160 * 1024) / (lds_bytes + kStaticLds));
     static constexpr int WAVES_PER_SIMD_RAW = WG_PER_CU * (NT / 64) / 4;
     static constexpr int WAVES_PER_SIMD = WAVES_PER_SIMD_RAW < 1 ? 1 : (WAVES_PER_SIMD_RAW > 4 ? 4 : WAVES_PER_SIMD_RAW);
 };
+
+// W_M^t for t < M, from double precision
+template <class T, int M>
+__device__ __forceinline__ void init_twiddles(cplx<T>* tw, int tid, int nt) {
+    for (int t = tid; t < M; t += nt) {
+        double s, c;
+        sincospi(2.0 * t / M, &s, &c);
+        tw[t] = cplx<T>{(T)c, (T)(-s)};
+    }
+}
+
+// The fit's least-squares rows (6 x kk doubles, kk = F^2 <= 81) in LDS: the loads are
+// issued at the start (clamped addresses, no branch around them), stored once the
+// first stage waits anyway, read by the fit at the end.
+template <int NT, int N>
+__device__ __forceinline__ void load_lsq(double (&v)[N], const double* lsq, int kk, int tid) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[q] = kk > 0 ? lsq[min(tid + q * NT, 6 * kk - 1)] : 0.0;
+}
+template <int NT, int N>
+__device__ __forceinline__ void store_lsq(double* dst, const double (&v)[N], int kk, int tid) {
+#pragma unroll
+    for (int q = 0; q < N; ++q)
+        if (tid + q * NT < 6 * kk) dst[tid + q * NT] = v[q];
+}
 
 // PAD: the slice side is below M/2 (a runtime value); otherwise it is M/2 and every
 // index expression below folds to constants.
@@ -89,7 +117,9 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
     __shared__ T sv[NT / kWave];
     __shared__ uint32_t si[NT / kWave];
     __shared__ T nbv[81];
+    __shared__ double lsq_s[6 * 81];
     __shared__ double cf[6];
+    constexpr int kLsqPer = (6 * 81 + NT - 1) / NT;  // least-squares elements per thread
 
     if (blockIdx.x >= a.total) return;
     const uint32_t sn = xcd_slice(blockIdx.x, a.total, a.nb);
@@ -102,29 +132,31 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
     // M-point C2R carries the reference's (2s)^2 scale; s^2 / M^2 is exact in T
     const T sc = (T)(0.25 * ((double)(2 * s) / M) * ((double)(2 * s) / M));
 
-    // W_M^t for t < M, from double precision
-    for (int t = tid; t < M; t += NT) {
-        double s, c;
-        sincospi(2.0 * t / M, &s, &c);
-        tw[t] = C{(T)c, (T)(-s)};
-    }
-
-    // ---- 0. prepared ROI into LDS
+    // ---- 0. prepared ROI into LDS; every global load of the stage is issued before
+    // the first use (the window factors and the fit's least-squares rows too)
+    const int kk = PIPE ? a.fit * a.fit : 0;
     if constexpr (FROM_PATTERN) {
         const size2 b0 = a.begins[bn];
         const IN* pic = a.pics + (size_t)(sn / a.nb) * a.src.area() + (size_t)b0.y * a.src.x + b0.x;
         constexpr int E = (S * S + NT - 1) / NT;
         IN px[E];
+        T wx[E], wy[E];
 #pragma unroll
         for (int it = 0; it < E; ++it) {
-            const int e = tid + it * NT;
-            px[it] = e < s * s ? pic[(size_t)(e / s) * a.src.x + (e % s)] : IN(0);
+            const int e = min(tid + it * NT, s * s - 1);  // clamped: masked below
+            px[it] = pic[(size_t)(e / s) * a.src.x + (e % s)];
+            wx[it] = a.hx[e % s];
+            wy[it] = a.hy[e / s];
         }
+        double lv[kLsqPer];
+        load_lsq<NT>(lv, a.lsq, kk, tid);
+        init_twiddles<T, M>(tw, tid, NT);
+        store_lsq<NT>(lsq_s, lv, kk, tid);
         T mean;
         if constexpr (std::is_same<IN, uint16_t>::value) {
             uint32_t su = 0;
 #pragma unroll
-            for (int it = 0; it < E; ++it) su += px[it];
+            for (int it = 0; it < E; ++it) su += tid + it * NT < s * s ? (uint32_t)px[it] : 0u;
 #pragma unroll
             for (int off = kWave / 2; off > 0; off >>= 1) su += __shfl_xor(su, off);
             if (tid % kWave == 0) red_u[tid / kWave] = su;
@@ -139,11 +171,15 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
 #pragma unroll
         for (int it = 0; it < E; ++it) {
             const int e = tid + it * NT;
-            if (e < s * s) pre[e] = prep_px<T, IN>(px[it], mean, a.hx[e % s], a.hy[e / s]);
+            if (e < s * s) pre[e] = prep_px<T, IN>(px[it], mean, wx[it], wy[it]);
         }
     } else {
         const T* src = a.prepared + (size_t)sn * s * s;
         for (int e = tid; e < s * s; e += NT) pre[e] = src[e];
+        double lv[kLsqPer];
+        load_lsq<NT>(lv, a.lsq, kk, tid);
+        init_twiddles<T, M>(tw, tid, NT);
+        store_lsq<NT>(lsq_s, lv, kk, tid);
     }
     __syncthreads();
 
@@ -330,7 +366,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
         bi = block_min_u32<NT>(bi, si);
 
         // ---- D. gather (FFT position policy, clamp to [0, 2s-1]) and fit
-        const int fit = a.fit, kk = fit * fit, r = (fit - 1) / 2;
+        const int fit = a.fit, r = (fit - 1) / 2;
         const size2 cross{s2 - 1, s2 - 1};
         const size2 mp = shift_pos(POLICY_FFT, bi, cross);
         for (int t = tid; t < kk; t += NT) {
@@ -347,7 +383,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
         __syncthreads();
         if (tid < 6) {
 #pragma clang fp contract(off)
-            const double* mrow = a.lsq + tid * kk;
+            const double* mrow = lsq_s + tid * kk;
             double sum = 0;
             for (int i = 0; i < kk; ++i) sum += mrow[i] * (double)nbv[i];
             cf[tid] = sum;
@@ -391,7 +427,7 @@ void launch_fft256(fft_args<float, IN, SUM> a, hipStream_t st);
 template <class T, class IN, class SUM, int M, bool FP, bool PIPE, bool PAD>
 void launch_fft_mp(const fft_args<T, IN, SUM>& a, uint32_t grid, hipStream_t st) {
     using P = fft_plan<T, M>;
-    static_assert(P::lds_bytes + 2048 <= kMaxLds, "FFT working set exceeds LDS");
+    static_assert(P::lds_bytes + kStaticLds <= kMaxLds, "FFT working set exceeds LDS");
     auto kern = fft_kernel<T, IN, SUM, M, FP, PIPE, PAD>;
     static const hipError_t attr =
         hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P::lds_bytes);
@@ -446,8 +482,7 @@ void launch_fft_ref(const IN* pics, const S* sums, const T* hx, const T* hy, con
 template <class T, class IN, class S>
 void launch_fft_pipe(const IN* pics, const S* sums, const T* hx, const T* hy, const size2* begins, size2 src,
                      uint32_t side, uint32_t nb, uint32_t total, const cplx<T>* spectra, int fit,
-                     data_index<T>* maxes, T* neigh, double* offsets, double* coefs, T* corr_out, void* work,
-                     hipStream_t st) {
+                     data_index<T>* maxes, T* neigh, double* offsets, double* coefs, T* corr_out, hipStream_t st) {
     fft_args<T, IN, S> a{};
     a.pics = pics;
     a.sums = sums;
@@ -465,16 +500,15 @@ void launch_fft_pipe(const IN* pics, const S* sums, const T* hx, const T* hy, co
     a.offsets = offsets;
     a.coefs = coefs;
     a.corr_out = corr_out;
-    a.work = (cplx<T>*)work;
     if (total == 0) return;
     launch_fft_side<T, IN, S, true, true>(side, a, total, st);
 }
 
 // Reference spectra from prepared reference slices, then the correlation of every
-// prepared pattern slice; `work` and `chunk` serve the 256-point path only.
+// prepared pattern slice.
 template <class T>
 static void fft_corr_prepared(const T* pics, const T* ref, T* res, uint32_t side, uint32_t ref_slices,
-                              uint32_t batch, cplx<T>* spectra, void* work, uint32_t chunk, hipStream_t st) {
+                              uint32_t batch, cplx<T>* spectra, hipStream_t st) {
     fft_args<T, T, T> a{};
     a.nb = ref_slices;
     a.total = ref_slices;
@@ -489,8 +523,6 @@ static void fft_corr_prepared(const T* pics, const T* ref, T* res, uint32_t side
     b.fit = 3;
     b.lsq = lstsq_device_table();
     b.corr_out = res;
-    b.work = (cplx<T>*)work;
-    b.chunk = chunk;
     launch_fft_side<T, T, T, false, true>(side, b, b.total, st);
 }
 
@@ -516,9 +548,7 @@ void launch_fft_corr_prepared(const T* pics, const T* ref, T* res, size2 side, u
         launch_fftg_corr_prepared<T>(pics, ref, res, side, ref_slices, batch, spectra, work, chunk, st);
     } else {
         EMIDA_HIP(hipMallocAsync((void**)&spectra, spectra_bytes<T>(side.x, ref_slices), st));
-        if (fft_padded(side.x) > 128)
-            EMIDA_HIP(hipMallocAsync(&work, fft256_work_bytes(ref_slices, ref_slices * batch), st));
-        fft_corr_prepared<T>(pics, ref, res, side.x, ref_slices, batch, spectra, work, 0, st);
+        fft_corr_prepared<T>(pics, ref, res, side.x, ref_slices, batch, spectra, st);
     }
     if (work) EMIDA_HIP(hipFreeAsync(work, st));
     EMIDA_HIP(hipFreeAsync(spectra, st));
@@ -560,7 +590,7 @@ void launch_fft_corr_prepared_noalloc(const T* pics, const T* ref, T* res, size2
         throw arg_error("cross_corr_fft without allocation: the reference spectra of " + std::to_string(ref_slices) +
                         " slices need " + std::to_string(spec) + " bytes, more than the " +
                         std::to_string(kFftStaticScratchBytes) + "-byte static scratch");
-    // the rest holds the work buffer of one launch chunk
+    // the rest holds the work buffer of the three-kernel path's launch chunks
     const size_t left = kFftStaticScratchBytes - spec;
     char* base = nullptr;
     if (general) {
@@ -573,19 +603,8 @@ void launch_fft_corr_prepared_noalloc(const T* pics, const T* ref, T* res, size2
         launch_fftg_corr_prepared<T>(pics, ref, res, side, ref_slices, batch, (cplx<T>*)base, base + spec, chunk, st);
         return;
     }
-    uint32_t chunk = 0;
-    if (fft_padded(side.x) > 128) {
-        // launch chunks of whole sets of ROIs whose work buffers fit the rest
-        chunk = (uint32_t)std::min<size_t>(left / fft256_slice_work_bytes() / ref_slices * ref_slices,
-                                           (size_t)ref_slices * batch);
-        if (chunk == 0)
-            throw arg_error("cross_corr_fft without allocation: " + std::to_string(ref_slices) + " slices of side " +
-                            std::to_string(side.x) + " need more than the " + std::to_string(kFftStaticScratchBytes) +
-                            "-byte static scratch");
-    }
     base = (char*)static_scratch();
-    fft_corr_prepared<T>(pics, ref, res, side.x, ref_slices, batch, (cplx<T>*)base, chunk ? base + spec : nullptr,
-                         chunk, st);
+    fft_corr_prepared<T>(pics, ref, res, side.x, ref_slices, batch, (cplx<T>*)base, st);
 }
 
 #define INST_CORR_PREPARED(T)                                                                                   \
@@ -619,7 +638,7 @@ void launch_hadamard(cplx<T>* pics, const cplx<T>* ref, size2 one, uint32_t ref_
                                            uint32_t, uint32_t, cplx<T>*, hipStream_t);                        \
     template void launch_fft_pipe<T, IN, S>(const IN*, const S*, const T*, const T*, const size2*, size2,     \
                                             uint32_t, uint32_t, uint32_t, const cplx<T>*, int,                \
-                                            data_index<T>*, T*, double*, double*, T*, void*, hipStream_t);
+                                            data_index<T>*, T*, double*, double*, T*, hipStream_t);       
 INST_FFT(float, uint16_t, uint32_t)
 INST_FFT(double, uint16_t, uint32_t)
 INST_FFT(float, float, float)

@@ -190,9 +190,6 @@ struct fft_args {
     double* offsets;
     double* coefs;
     T* corr_out;             // optional (2s)^2 circular buffers in the reference layout, row-major
-    cplx<T>* work;           // two-kernel path: column-stage results of this launch's slices
-    uint32_t first;          // two-kernel path: first slice of this launch (a multiple of nb)
-    uint32_t chunk;          // two-kernel path: slices per launch chunk (a multiple of nb; 0: default)
 };
 
 }  // namespace emida_amd

@@ -73,17 +73,12 @@ void mfma_refine_bounds(int mode, size2 s, bool t_double, double& tol_rel, doubl
 
 // ---- fft.hip, fft256.hip
 // Largest padded FFT side of the single-kernel pipelines: 128 for double, 256 for
-// float (M = 256 runs as two kernels with a work buffer, see fft256.hip); fftg.hip
-// takes every other shape up to kFftMaxSlice per axis.
+// float (M = 256 in fft256.hip); fftg.hip takes every other shape up to kFftMaxSlice
+// per axis.
 int fft_max_side(bool is_double);
 constexpr uint32_t kFftMaxSlice = 256;
 // Transform side M of a slice side s: the power of two >= 2s (at least 4).
 uint32_t fft_padded(uint32_t side);
-// M = 256: slices per launch chunk and the work buffer a launch of `total` slices needs.
-constexpr uint32_t kFft256Chunk = 4096;
-size_t fft256_chunk(uint32_t nb);
-size_t fft256_work_bytes(uint32_t nb, uint32_t total);
-size_t fft256_slice_work_bytes();  // work buffer bytes per slice
 // Reference spectra: prepared reference slices -> half spectra in the layout the
 // pipeline kernel reads ((M/2+1) columns x M bit-reversed rows per ROI).
 template <class T, class IN, class S>
@@ -91,21 +86,19 @@ void launch_fft_ref(const IN* pics, const S* sums, const T* hx, const T* hy, con
                     uint32_t side, uint32_t nb, cplx<T>* spectra, hipStream_t st);
 // Whole FFT pipeline of `total` slices (patterns x ROIs): prepare, R2C, multiply by
 // conj(reference spectrum), C2R, arg-max, gather, fit.  corr_out (optional)
-// receives the (2s)^2 circular correlation buffers; work (M = 256 only) holds
-// fft256_work_bytes(nb, total) bytes.
+// receives the (2s)^2 circular correlation buffers.
 template <class T, class IN, class S>
 void launch_fft_pipe(const IN* pics, const S* sums, const T* hx, const T* hy, const size2* begins, size2 src,
                      uint32_t side, uint32_t nb, uint32_t total, const cplx<T>* spectra, int fit,
-                     data_index<T>* maxes, T* neigh, double* offsets, double* coefs, T* corr_out, void* work,
-                     hipStream_t st);
+                     data_index<T>* maxes, T* neigh, double* offsets, double* coefs, T* corr_out, hipStream_t st);
 // Prepared slices in, circular correlation buffers out (cross_corr_fft seam).
 template <class T>
 void launch_fft_corr_prepared(const T* pics, const T* ref, T* res, size2 side, uint32_t ref_slices, uint32_t batch,
                               hipStream_t st);
 // The same without any allocation: the reference spectra (and the work buffer of the
-// 256-point path) live in the library's static device array of kFftStaticScratchBytes
+// three-kernel path) live in the library's static device array of kFftStaticScratchBytes
 // per GPU (shared by every call on that GPU: overlapping calls race).  Throws
-// arg_error when the spectra (plus one work chunk of ref_slices slices) exceed it.
+// arg_error when the spectra (plus one slice's work buffer) exceed it.
 constexpr size_t kFftStaticScratchBytes = (size_t)32 << 20;
 template <class T>
 void launch_fft_corr_prepared_noalloc(const T* pics, const T* ref, T* res, size2 side, uint32_t ref_slices,

@@ -226,8 +226,6 @@ struct emida_offset_s {
             parts[5] = al256(tn * cross.y * sizeof(uint32_t));
         } else if (fftg) {
             parts[2] = al256(fftg_work_bytes(T == EMIDA_F64, slice, (uint32_t)tn, true));  // half spectra + corr
-        } else if (fft_padded(side) > 128) {
-            parts[2] = al256(fft256_work_bytes(nb, (uint32_t)tn));  // column-stage work buffer
         }
         return parts[0] + parts[1] + parts[2] + parts[3] + parts[4] + parts[5];
     }
@@ -444,7 +442,6 @@ struct emida_offset_s {
             const size_t m = fft_padded(side);
             const size_t spec = (m / 2 + 1) * m * 2 * t_bytes();
             d_ref = alloc(spec * nb);
-            if (m > 128 && !resident_only) d_corr = alloc(fft256_work_bytes(nb, total));
         }
         const size_t nk = (size_t)fit * fit;
         for (auto& o : out) {
@@ -627,8 +624,7 @@ struct emida_offset_s {
                     else
                         launch_fft_pipe<TT, IT, ST>(pics, sums, (const TT*)d_hx, (const TT*)d_hy, d_begins, src,
                                                     side, nb, tn, (const cplx<TT>*)d_ref, fit,
-                                                    (data_index<TT>*)maxes, (TT*)neigh, offsets, coefs, nullptr,
-                                                    sc.corr, st);
+                                                    (data_index<TT>*)maxes, (TT*)neigh, offsets, coefs, nullptr, st);
                     if (timing) record_stop(st, t0);
                 }
             });

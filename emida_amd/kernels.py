@@ -96,7 +96,17 @@ def run_cross_corr_fft(pics, ref, size, ref_slices=1, batch_size=1):
 def run_cross_corr_fft_device(d_pics, d_ref, d_res, dtype, size, ref_slices=1, batch_size=1, noalloc=False,
                               stream=None):
     """cross_corr_fft on device buffers already in place (no upload or download);
-    noalloc: the allocation-free entry point, working in the library's static scratch."""
+    noalloc: the allocation-free entry point, working in the library's static scratch.
+    Buffers with a known size (DeviceBuffer) are checked against what the call reads
+    and writes before anything runs."""
+    sx, sy = (size, size) if np.isscalar(size) else size
+    item = np.dtype(dtype).itemsize
+    n_ref = int(sx) * int(sy) * int(ref_slices) * item
+    for buf, need, what in ((d_pics, n_ref * batch_size, "pattern slices"), (d_ref, n_ref, "reference slices"),
+                            (d_res, 4 * n_ref * batch_size, "result buffers")):
+        have = getattr(buf, "nbytes", None)
+        if have is not None and have < need:
+            raise ValueError(f"cross_corr_fft: the {what} need {need} bytes, the buffer holds {have}")
     fn = C.lib().emida_run_cross_corr_fft_noalloc if noalloc else C.lib().emida_run_cross_corr_fft
     C.check(fn(C.type_code(np.dtype(dtype)), d_pics.ptr, d_ref.ptr, d_res.ptr, C.size2(size), ref_slices, batch_size,
                None if stream is None else ctypes.c_void_p(int(stream))))

@@ -155,3 +155,17 @@ def test_entry_points_validate_arguments_before_any_device_call():
     assert rc == C.ERR_ARG and "cross size" in m
     rc, m = err(L.emida_run_cross_corr(1, d, d, d, C.Size2(4, 4), C.Size2(10, 9), 1, 1, None))
     assert rc == C.ERR_ARG and "at most 2*size+1" in m
+
+
+def test_cross_corr_fft_device_checks_buffer_sizes_first():
+    # buffers of known size smaller than the call reads or writes are refused on the
+    # host, before the library is called (this runs without a GPU)
+    class Buf:
+        def __init__(self, nbytes):
+            self.nbytes, self.ptr = nbytes, ctypes.c_void_p(1)
+
+    n = 128 * 128 * 3 * 2 * 4  # float, 3 ROIs x 2 patterns of 128 x 128
+    for pics, ref, res, what in ((n - 4, n, 4 * n, "pattern slices"), (n, n // 2 - 4, 4 * n, "reference slices"),
+                                 (n, n, 4 * n - 4, "result buffers")):
+        with pytest.raises(ValueError, match=what):
+            K.run_cross_corr_fft_device(Buf(pics), Buf(ref), Buf(res), np.float32, (128, 128), 3, 2)

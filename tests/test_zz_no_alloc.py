@@ -19,8 +19,8 @@ CASES = [
     (np.float32, 32, 3, 4, (2, 3)),
     (np.float64, 64, 2, 3, (1, 2)),
     (np.float32, 20, 4, 2, (3, 1)),
-    (np.float32, 96, 5, 3, (4, 2)),     # 256-point path, one work chunk
-    (np.float32, 128, 40, 3, (36, 2)),  # 256-point path, work in chunks of whole ROI sets
+    (np.float32, 96, 5, 3, (4, 2)),     # 256-point path
+    (np.float32, 128, 40, 3, (36, 2)),  # 256-point path, 10 MiB of reference spectra
     (np.float64, 128, 4, 3, (3, 1)),    # three-kernel path (double 256-point transforms)
     (np.float32, (64, 48), 5, 2, (2, 2)),  # three-kernel path (non-square)
     (np.float64, 256, 12, 2, (5, 1)),   # three-kernel path, 512-point transforms in several chunks
@@ -52,8 +52,12 @@ def test_noalloc_equals_allocating_entry_point(T, side, nb, npat, second):
 def test_noalloc_refuses_sizes_beyond_its_scratch():
     cap = K.fft_noalloc_scratch_bytes()
     assert cap == 32 << 20
-    one = emida_amd.DeviceBuffer(16)
-    for T, side, nb in [(np.float64, 64, 300), (np.float32, 128, 100), (np.float64, 256, 16)]:
+    # reference spectra of 40 MiB (double, 128-point), 52 MiB (float, 256-point) and the
+    # three-kernel path's spectra plus work buffer; the buffers have the sizes the call
+    # reads and writes, so a call that were not refused would still stay in bounds
+    for T, side, nb in [(np.float64, 64, 300), (np.float32, 128, 200), (np.float64, 256, 16)]:
+        n = side * side * nb * np.dtype(T).itemsize
+        pics, ref, res = emida_amd.DeviceBuffer(n), emida_amd.DeviceBuffer(n), emida_amd.DeviceBuffer(4 * n)
         with pytest.raises(emida_amd.EmidaError) as e:
-            K.run_cross_corr_fft_device(one, one, one, T, (side, side), nb, 1, noalloc=True)
+            K.run_cross_corr_fft_device(pics, ref, res, T, (side, side), nb, 1, noalloc=True)
         assert e.value.code == emida_amd._capi.ERR_ARG and "static scratch" in str(e.value)

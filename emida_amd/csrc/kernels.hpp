@@ -1,6 +1,8 @@
 // Host-side launchers of the gfx950 kernels (internal to the library).
 #pragma once
 
+#include <cmath>
+
 #include "common.hpp"
 
 namespace emida_amd {
@@ -14,14 +16,22 @@ void launch_prepare(const IN* pics, T* slices, const T* hx, const T* hy, const S
                     size2 src, size2 slice, size2 out, uint32_t nb, uint32_t total, hipStream_t st);
 // Sums and preparation of uint16 slices in one kernel (sum_kernel + prepare_kernel +
 // prepare_split_kernel with identical results); slices (T, slice layout) and hi/lo
-// (half planes scaled by in_scale) are optional; zero (optional) gets zrow zeros per
-// slice.  Returns false, launching nothing, when a slice does not fit the LDS staging
-// (kSumPrepareMaxLds bytes of pixels).
+// (half planes scaled by in_scale) are optional.  stats (optional, with slices and the
+// prepared T reference slices ref [roi][slice]): per slice, in double, the sum of
+// squares of the prepared slice (stats[slice]) and its sum of products with the
+// reference slice (stats[total + slice]).  Returns false, launching nothing, when a
+// slice does not fit the LDS staging (kSumPrepareMaxLds bytes of pixels).
 constexpr size_t kSumPrepareMaxLds = 32768;
 template <class T, class HT>
 bool launch_sum_prepare(const uint16_t* pics, uint32_t* sums, T* slices, const T* hx, const T* hy, HT* hi, HT* lo,
-                        const float* hxf, const float* hyf, float in_scale, uint32_t* zero, uint32_t zrow,
+                        const float* hxf, const float* hyf, float in_scale, double* stats, const T* ref,
                         const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st);
+// The same statistics of n prepared slices of `area` T values: sums of squares into
+// out[0..n) and, with ref (slice i against ref slice i % nb), sums of products into
+// out[n..2n).
+template <class T>
+void launch_slice_stats(const T* slices, const T* ref, uint32_t area, uint32_t nb, uint32_t n, double* out,
+                        hipStream_t st);
 
 // ---- brute.hip
 // res[(i*ref_slices + j)][...] = correlation of pics slice (i, j) with ref slice j.
@@ -57,16 +67,45 @@ template <class T>
 void launch_subpixel(const T* neigh, int s, uint32_t n, const data_index<T>* maxes, size2 cross, int policy,
                      double* offsets, double* coefs, hipStream_t st);
 
+// Candidates of the exact refinement as the matrix-core kernel publishes them in
+// place of the screen: per (slice, row), every position whose screen value is within
+// tau of a lower bound of the slice maximum (the largest row maximum the publishing
+// workgroup holds, or the zero shift's dot product less tau), so each candidate of the
+// refinement is among them.
+constexpr int kListCap = 32;  // positions kept per row; a row with more makes its slice sweep
+struct cand_entry {
+    float v;     // screen value
+    uint32_t x;  // column of the screen row
+};
+struct screen_lists {
+    uint32_t* cnt;      // [slice][row]: number of such positions (may exceed kListCap)
+    cand_entry* ent;    // [slice][row][kListCap]: the first kListCap of them, in no order
+    const double* pn2;  // [slice]: sum of squares of the T-precision prepared pattern slice
+    const double* rn2;  // [roi]: the same for the reference slice
+    const double* z0;   // [slice]: sum of p * r over the slice (the zero shift), in double
+    double tol_rel, tol_abs;
+};
+// tau = tol_rel ||p|| ||r|| + tol_abs (||p|| + ||r||) of one slice, evaluated the same
+// way wherever it is used.
+__host__ __device__ inline double refine_tau(const screen_lists& L, size_t slice, uint32_t roi) {
+#pragma clang fp contract(off)
+    const double np = sqrt(L.pn2[slice]), nr = sqrt(L.rn2[roi]);
+    return L.tol_rel * np * nr + L.tol_abs * (np + nr);
+}
+
 // Precision form: exact peak from a float matrix-core screen of `total` slices and the
 // T-precision prepared pattern (p, [slice][s]) and reference (r, [roi][s]) slices;
 // outputs as launch_peak with the brute position policy.  The candidate bound is
 // tau = tol_rel ||p|| ||r|| + tol_abs (||p|| + ||r||) per slice.  screen_max (optional):
-// the screen maximum of each slice as an ordered code (screen_max_code), from
-// launch_mfma_corr; without it the kernel scans the screen for its maximum.
+// the screen maximum of each slice row as an ordered code (screen_max_code), from
+// launch_mfma_corr; without it the kernel scans the screen for its maximum.  lists
+// (optional, with screen_max): the published candidates; the screen is then not read
+// (it may be null) and tau comes from the lists' norms.
 template <class T>
 void launch_refine(const float* screen, size_t corr_len, const T* p, const T* r, size2 s, size2 cross, uint32_t nb,
                    double tol_rel, double tol_abs, int fit, data_index<T>* maxes, T* neigh, double* offsets,
-                   double* coefs, uint32_t total, hipStream_t st, const uint32_t* screen_max = nullptr);
+                   double* coefs, uint32_t total, hipStream_t st, const uint32_t* screen_max = nullptr,
+                   const screen_lists* lists = nullptr);
 // Screen error bound of launch_mfma_corr against the T-precision brute values, for
 // launch_refine (both values' rounding, counted twice): mode as emida_mfma_mode.
 void mfma_refine_bounds(int mode, size2 s, bool t_double, double& tol_rel, double& tol_abs);
@@ -140,19 +179,18 @@ void launch_fftg_std_corr(const T* pic, const cplx<T>* spectra, T* res, size2 si
 
 // ---- mfma.hip (matrix-core brute correlation, float results)
 // Prepared slices as half-type hi (+ lo when lo != nullptr) planes [slice][y][x];
-// corr: [pattern * nb + roi][cross.y][cross.x].  screen_max (optional, [pattern * nb +
-// roi][cross.y], zero on entry): receives the largest value of each row of each slice
-// (NaN ignored) as an ordered code, atomically.
+// corr (optional): the screen, [pattern * nb + roi][cross.y][cross.x].  screen_max
+// (optional, [pattern * nb + roi][cross.y]): the largest value of each row of each
+// slice (NaN ignored) as an ordered code, 0 for a row without one.  lists (optional,
+// with screen_max): the refinement's candidates per row (screen_lists).
 template <class HT>
 void launch_mfma_corr(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
-                      uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* screen_max = nullptr);
+                      uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* screen_max = nullptr,
+                      const screen_lists* lists = nullptr);
 void check_mfma_shape(size2 s, size2 cross);
-// zero (optional): zrow uint32 per slice set to 0 (the screen row maxima of the next
-// launch_mfma_corr).
 template <class IN, class S, class HT>
 void launch_prepare_split(const IN* pics, HT* hi, HT* lo, const float* hx, const float* hy, const S* sums,
-                          const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st,
-                          uint32_t* zero = nullptr, uint32_t zrow = 0);
+                          const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st);
 
 // Order-preserving uint32 code of a float (a < b  <=>  code(a) < code(b), -0 < +0);
 // 0 is below every code and stands for "no value".

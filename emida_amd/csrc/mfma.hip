@@ -65,8 +65,7 @@ __global__ __launch_bounds__(256) void prepare_split_kernel(const IN* __restrict
                                                             HT* __restrict__ lo, const float* __restrict__ hx,
                                                             const float* __restrict__ hy, const S* __restrict__ sums,
                                                             const size2* __restrict__ begins, size2 src, size2 slice,
-                                                            uint32_t nb, uint32_t total, uint32_t* __restrict__ zero,
-                                                            uint32_t zrow) {
+                                                            uint32_t nb, uint32_t total) {
 #pragma clang fp contract(off)
     // blocks_per_slice blocks per slice: wave-uniform 32-bit slice index
     const uint32_t area = slice.area(), bps = div_up(area, 256u);
@@ -74,7 +73,6 @@ __global__ __launch_bounds__(256) void prepare_split_kernel(const IN* __restrict
     const uint32_t st = (blockIdx.x - sn * bps) * 256u + threadIdx.x;
     if (sn >= total || st >= area) return;
     const size_t tid = (size_t)sn * area + st;
-    if (zero && st < zrow) zero[(size_t)sn * zrow + st] = 0u;
     const uint32_t x = st % slice.x, y = st / slice.x;
     const size2 b0 = begins[sn % nb];
     float v = (float)pics[(size_t)(sn / nb) * src.area() + (size_t)(b0.y + y) * src.x + b0.x + x];
@@ -126,7 +124,7 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
                                                                 const HT* __restrict__ rlo,
                                                                 float* __restrict__ corr, size2 s, size2 cross,
                                                                 uint32_t nb, uint32_t npat, uint32_t n_wg,
-                                                                uint32_t* __restrict__ smax) {
+                                                                uint32_t* __restrict__ smax, screen_lists lists) {
     using P = mfma_plan<HT, SPLIT, KMAX, WX>;
     using V = typename mfma_vec<HT>::type;
     constexpr int NT = kMfmaNT, TS = kMfmaTS, ROWS = P::ROWS;
@@ -294,13 +292,24 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
         }
         __syncthreads();
     }
-    // per-row maxima of the values written below (NaN never wins a strict compare): for
-    // each y-shift, per accumulator row j the lane's largest value over its two tiles,
-    // then over the 32 lanes of the half by a reduce-scatter (xor 16, 8, 4, 2 halve the
-    // rows a lane keeps, xor 1 pairs the last one: 16 lane exchanges instead of 80), so
-    // lane l ends with row j = 8 b4 + 4 b3 + 2 b2 + b1 (bits of l) and the even lane of
-    // each pair publishes pattern row 8 (j/4) + 4 (l/32) + j%4 with one atomic max
+    // ---- epilogue.  The workgroup holds complete screen rows (every x-shift of its
+    // y-shifts and patterns), so a row's maximum is final here.  Per-row maxima (NaN
+    // never wins a strict compare): for each y-shift, per accumulator row j the lane's
+    // largest value over its two tiles, then over the 32 lanes of the half by a
+    // reduce-scatter (xor 16, 8, 4, 2 halve the rows a lane keeps, xor 1 pairs the last
+    // one: 16 lane exchanges instead of 80), so lane l ends with row j = 8 b4 + 4 b3 +
+    // 2 b2 + b1 (bits of l) and the even lane of each pair folds pattern row
+    // 8 (j/4) + 4 (l/32) + j%4 into the workgroup's LDS maximum of that screen row
+    // (over the WX waves of the row).  Stored per (slice, row); with lists, every value
+    // within tau of the largest row maximum of the workgroup's rows of its pattern is
+    // published as well (a superset of the candidates: the slice maximum is at least that).
     if (smax) {
+        __syncthreads();  // the row ring is free
+        uint32_t* lcode = reinterpret_cast<uint32_t*>(lds_raw);  // [ROWS][32] row maximum codes
+        uint32_t* lcnt = lcode + ROWS * kMfmaM;                  // [ROWS][32] published positions
+        double* lthr = reinterpret_cast<double*>(lcnt + ROWS * kMfmaM);  // [ROWS][32] thresholds
+        for (int i = tid; i < ROWS * kMfmaM; i += NT) lcode[i] = lcnt[i] = 0u;
+        __syncthreads();
         auto keep_max = [](float& k, float o) {
             if (o > k) k = o;
         };
@@ -333,12 +342,67 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
             }
             keep_max(mx[0], __shfl_xor(mx[0], 1));
             const int j = 8 * ((lane >> 4) & 1) + 4 * ((lane >> 3) & 1) + 2 * ((lane >> 2) & 1) + ((lane >> 1) & 1);
-            if (!(lane & 1) && mx[0] > -__builtin_huge_valf()) {
-                const uint32_t p = p0 + 8 * (j / 4) + 4 * half + (j % 4);
-                if (p < npat) atomicMax(&smax[((size_t)p * nb + roi) * cross.y + row], screen_max_code(mx[0]));
+            if (!(lane & 1) && mx[0] > -__builtin_huge_valf())
+                atomicMax(&lcode[(TS * wy + t) * kMfmaM + 8 * (j / 4) + 4 * half + (j % 4)], screen_max_code(mx[0]));
+        }
+        __syncthreads();
+        for (int i = tid; i < ROWS * kMfmaM; i += NT) {
+            const int row = sy0 + i / kMfmaM + ry;
+            const uint32_t p = p0 + i % kMfmaM;
+            if (row >= (int)cross.y || p >= npat) continue;
+            const size_t sl = (size_t)p * nb + roi;
+            const uint32_t code = lcode[i];
+            smax[sl * cross.y + row] = code;
+            if (lists.cnt) {
+                // the slice maximum is at least the largest row maximum of this
+                // workgroup's rows of the pattern and at least z0 - tau (z0: the
+                // pattern's dot product with the reference at the zero shift, whose
+                // screen value is within tau/2 of it): positions more than tau below
+                // that bound cannot be candidates
+                uint32_t best = 0;
+                for (int r = 0; r < ROWS; ++r) best = max(best, lcode[r * kMfmaM + i % kMfmaM]);
+                const double tau = refine_tau(lists, sl, roi);
+                const double lb = fmax(best ? (double)screen_max_value(best) : -__builtin_huge_val(),
+                                       lists.z0[sl] - tau);
+                lthr[i] = best ? lb - tau : __builtin_huge_val();
+            }
+        }
+        if (lists.cnt) {
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < TS; ++t) {
+                const int row = syw + t + ry;
+                if (row >= (int)cross.y) continue;
+#pragma unroll
+                for (int u = 0; u < kMfmaTilesPerWave; ++u) {
+                    const int x = sx0[u] + m_a + rx;
+                    if (x >= (int)cross.x) continue;
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const int m = 8 * (j / 4) + 4 * half + (j % 4);
+                        const uint32_t p = p0 + m;
+                        if (p >= npat) continue;
+                        const int li = (TS * wy + t) * kMfmaM + m;
+                        const float v = acc[t][u][j] * mfma_vec<HT>::out;
+                        if ((double)v >= lthr[li]) {
+                            const uint32_t c = atomicAdd(&lcnt[li], 1u);
+                            if (c < (uint32_t)kListCap)
+                                lists.ent[(((size_t)p * nb + roi) * cross.y + row) * kListCap + c] =
+                                    cand_entry{v, (uint32_t)x};
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            for (int i = tid; i < ROWS * kMfmaM; i += NT) {
+                const int row = sy0 + i / kMfmaM + ry;
+                const uint32_t p = p0 + i % kMfmaM;
+                if (row >= (int)cross.y || p >= npat) continue;
+                lists.cnt[((size_t)p * nb + roi) * cross.y + row] = lcnt[i];
             }
         }
     }
+    if (!corr) return;
     // write the accumulators: D[8 (j/4) + 4 (l/32) + j%4][l%32]
 #pragma unroll
     for (int t = 0; t < TS; ++t) {
@@ -363,9 +427,10 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
 // --------------------------------------------------------------- launchers
 template <class HT, bool SPLIT, int KMAX, int WX>
 void launch_mfma_corr_g(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
-                        uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* smax) {
+                        uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* smax, const screen_lists& lists) {
     using P = mfma_plan<HT, SPLIT, KMAX, WX>;
     static_assert(P::lds_bytes <= 80 * 1024, "two workgroups per CU");
+    static_assert((size_t)P::ROWS * kMfmaM * 16 <= P::lds_bytes, "epilogue row tables exceed the LDS");
     auto kern = mfma_corr_kernel<HT, SPLIT, KMAX, WX>;
     static const hipError_t attr =
         hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P::lds_bytes);
@@ -374,20 +439,20 @@ void launch_mfma_corr_g(const HT* phi, const HT* plo, const HT* rhi, const HT* r
     const uint64_t n_wg = (uint64_t)nb * div_up(npat, (uint32_t)kMfmaM) * div_up(cross.y, (uint32_t)P::ROWS);
     if (n_wg > (1u << 24)) throw unsupported_error("matrix-core brute policy: too many workgroups in one launch");
     kern<<<(uint32_t)n_wg, kMfmaNT, P::lds_bytes, st>>>(phi, plo, rhi, rlo, corr, s, cross, nb, npat,
-                                                        (uint32_t)n_wg, smax);
+                                                        (uint32_t)n_wg, smax, lists);
     EMIDA_LAUNCHED("mfma_corr_kernel");
 }
 
 // Row width class of a slice: 32, 64 or 128 columns (windows up to 64, 128, 256 wide).
 template <class HT, bool SPLIT>
 void launch_mfma_corr_t(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
-                        uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* smax) {
+                        uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* smax, const screen_lists& lists) {
     if (s.x <= 32)
-        launch_mfma_corr_g<HT, SPLIT, 32, 1>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax);
+        launch_mfma_corr_g<HT, SPLIT, 32, 1>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax, lists);
     else if (s.x <= 64)
-        launch_mfma_corr_g<HT, SPLIT, 64, 2>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax);
+        launch_mfma_corr_g<HT, SPLIT, 64, 2>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax, lists);
     else
-        launch_mfma_corr_g<HT, SPLIT, 128, 4>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax);
+        launch_mfma_corr_g<HT, SPLIT, 128, 4>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax, lists);
 }
 
 void check_mfma_shape(size2 s, size2 cross) {
@@ -399,13 +464,18 @@ void check_mfma_shape(size2 s, size2 cross) {
 
 template <class HT>
 void launch_mfma_corr(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
-                      uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* smax) {
+                      uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* smax, const screen_lists* lists) {
     check_mfma_shape(s, cross);
+    screen_lists none{};
+    if (lists && (!smax || !lists->cnt || !lists->ent || !lists->pn2 || !lists->rn2 || !lists->z0))
+        throw arg_error("matrix-core brute policy: candidate lists need the row maxima, counts, entries and norms");
+    if (!corr && !smax) throw arg_error("matrix-core brute policy: no output");
     if (nb == 0 || npat == 0) return;
+    const screen_lists& L = lists ? *lists : none;
     if (plo)
-        launch_mfma_corr_t<HT, true>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax);
+        launch_mfma_corr_t<HT, true>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax, L);
     else
-        launch_mfma_corr_t<HT, false>(phi, nullptr, rhi, nullptr, corr, s, cross, nb, npat, st, smax);
+        launch_mfma_corr_t<HT, false>(phi, nullptr, rhi, nullptr, corr, s, cross, nb, npat, st, smax, L);
 }
 
 // |screen - T-precision brute value| <= tol_rel ||p|| ||r|| + tol_abs (||p|| + ||r||)
@@ -436,14 +506,12 @@ void mfma_refine_bounds(int mode, size2 s, bool t_double, double& tol_rel, doubl
 
 template <class IN, class S, class HT>
 void launch_prepare_split(const IN* pics, HT* hi, HT* lo, const float* hx, const float* hy, const S* sums,
-                          const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st,
-                          uint32_t* zero, uint32_t zrow) {
+                          const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st) {
     const size_t blocks = (size_t)div_up(slice.area(), 256u) * total;
-    if (zero && zrow > slice.area()) throw arg_error("prepare: more screen rows than slice pixels");
     if (blocks == 0) return;
     if (blocks > 0xffffffffu) throw arg_error("prepare: too many slices for one launch");
     prepare_split_kernel<IN, S, HT><<<(unsigned)blocks, 256, 0, st>>>(pics, hi, lo, hx, hy, sums, begins, src, slice,
-                                                                     nb, total, zero, zrow);
+                                                                     nb, total);
     EMIDA_LAUNCHED("prepare_split_kernel");
 }
 
@@ -474,18 +542,17 @@ void launch_split(const float* v, HT* hi, HT* lo, size_t n, hipStream_t st) {
 
 #define INST_MFMA(HT)                                                                                          \
     template void launch_mfma_corr<HT>(const HT*, const HT*, const HT*, const HT*, float*, size2, size2, uint32_t, \
-                                       uint32_t, hipStream_t, uint32_t*);                                      \
+                                       uint32_t, hipStream_t, uint32_t*, const screen_lists*);                 \
     template void launch_split<HT>(const float*, HT*, HT*, size_t, hipStream_t);                               \
     template void launch_prepare_split<uint16_t, uint32_t, HT>(const uint16_t*, HT*, HT*, const float*,        \
                                                                const float*, const uint32_t*, const size2*, size2, \
-                                                               size2, uint32_t, uint32_t, hipStream_t, uint32_t*,  \
-                                                               uint32_t);                                          \
+                                                               size2, uint32_t, uint32_t, hipStream_t);            \
     template void launch_prepare_split<float, float, HT>(const float*, HT*, HT*, const float*, const float*,    \
                                                          const float*, const size2*, size2, size2, uint32_t,    \
-                                                         uint32_t, hipStream_t, uint32_t*, uint32_t);          \
+                                                         uint32_t, hipStream_t);                               \
     template void launch_prepare_split<double, double, HT>(const double*, HT*, HT*, const float*, const float*, \
                                                            const double*, const size2*, size2, size2, uint32_t, \
-                                                           uint32_t, hipStream_t, uint32_t*, uint32_t);
+                                                           uint32_t, hipStream_t);
 INST_MFMA(__bf16)
 INST_MFMA(_Float16)
 

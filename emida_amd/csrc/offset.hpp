@@ -162,6 +162,12 @@ struct emida_offset_s {
     void* d_pic_hi = nullptr;  // matrix-core policy: prepared pattern slices, hi / lo halves
     void* d_pic_lo = nullptr;
     void* d_smax = nullptr;    // matrix-core policy: screen maximum per slice row
+    // matrix-core policy with refinement: the candidates published in place of the
+    // screen (screen_lists) and the norms of the T-precision slices
+    void* d_cnt = nullptr;
+    void* d_ent = nullptr;
+    void* d_pn2 = nullptr;
+    void* d_rn2 = nullptr;
     // the reference's cross_corr_fft seam (get_cu_pic / get_cu_temp / cross_corr_fft,
     // src/gpu_offset.hpp:303-325), FFT policy: made on first use
     void* d_seam_pic = nullptr;   // total x (2s.y rows of 2s.x) T
@@ -209,44 +215,58 @@ struct emida_offset_s {
         void* pic_hi = nullptr;
         void* pic_lo = nullptr;
         void* smax = nullptr;
+        void* cnt = nullptr;
+        void* ent = nullptr;
+        void* pn2 = nullptr;
     };
+    static constexpr int kParts = 9;
     static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
     size_t scratch_sizes(uint32_t n_pat, size_t* parts) const {
         const size_t tn = (size_t)nb * n_pat;
         parts[0] = al256(std::max<size_t>(tn, nb) * sum_bytes);
-        parts[1] = parts[2] = parts[3] = parts[4] = parts[5] = 0;
+        for (int i = 1; i < kParts; ++i) parts[i] = 0;
         if (policy == EMIDA_CROSS_POLICY_BRUTE) {
             parts[1] = al256(tn * slice.area() * t_bytes());
             parts[2] = al256(tn * cross.area() * t_bytes());
         } else if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA) {
-            if (mfma_refine) parts[1] = al256(tn * slice.area() * t_bytes());
-            parts[2] = al256(tn * cross.area() * sizeof(float));
+            if (mfma_refine) {
+                parts[1] = al256(tn * slice.area() * t_bytes());
+                parts[6] = al256(tn * cross.y * sizeof(uint32_t));
+                parts[7] = al256(tn * cross.y * kListCap * sizeof(cand_entry));
+                parts[8] = al256(tn * 2 * sizeof(double));  // norms, then zero-shift products
+            } else {
+                parts[2] = al256(tn * cross.area() * sizeof(float));  // the screen
+            }
             parts[3] = al256(tn * slice.area() * 2);
             if (mfma_split(mfma_mode)) parts[4] = parts[3];
             parts[5] = al256(tn * cross.y * sizeof(uint32_t));
         } else if (fftg) {
             parts[2] = al256(fftg_work_bytes(T == EMIDA_F64, slice, (uint32_t)tn, true));  // half spectra + corr
         }
-        return parts[0] + parts[1] + parts[2] + parts[3] + parts[4] + parts[5];
+        size_t sum = 0;
+        for (int i = 0; i < kParts; ++i) sum += parts[i];
+        return sum;
     }
     size_t scratch_bytes(uint32_t n_pat) const {
-        size_t parts[6];
+        size_t parts[kParts];
         return scratch_sizes(n_pat, parts);
     }
     scratch_t carve(void* base, uint32_t n_pat) const {
-        size_t parts[6];
+        size_t parts[kParts];
         scratch_sizes(n_pat, parts);
         char* b = (char*)base;
-        void* ptr[6];
-        for (int i = 0; i < 6; ++i) {
+        void* ptr[kParts];
+        for (int i = 0; i < kParts; ++i) {
             ptr[i] = parts[i] ? b : nullptr;
             b += parts[i];
         }
-        return {ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5]};
+        return {ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5], ptr[6], ptr[7], ptr[8]};
     }
     const void* float_hx() const { return d_hxf ? d_hxf : d_hx; }
     const void* float_hy() const { return d_hyf ? d_hyf : d_hy; }
-    scratch_t own_scratch() const { return {d_sums, d_slices, d_corr, d_pic_hi, d_pic_lo, d_smax}; }
+    scratch_t own_scratch() const {
+        return {d_sums, d_slices, d_corr, d_pic_hi, d_pic_lo, d_smax, d_cnt, d_ent, d_pn2};
+    }
 
     size_t ti_bytes() const { return T == EMIDA_F32 ? 8 : 16; }
     size_t t_bytes() const { return T == EMIDA_F32 ? 4 : 8; }
@@ -268,6 +288,7 @@ struct emida_offset_s {
         for (auto e : ev_done) (void)hipEventDestroy(e);
         void* ptrs[] = {d_begins, d_hx,     d_hy,     d_sums,  d_ref, d_slices, d_corr,
                         d_ref_lo, d_pic_hi, d_pic_lo, d_ref_t, d_hxf, d_hyf,    d_smax,
+                        d_cnt,    d_ent,    d_pn2,    d_rn2,
                         d_seam_pic, d_seam_temp, d_seam_res, d_seam_work};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
@@ -351,9 +372,12 @@ struct emida_offset_s {
                                                          (const float*)float_hy(), sums, d_begins, src, slice, nb, nb,
                                                          compute);
                     });
-                    if (mfma_refine)
+                    if (mfma_refine) {
                         launch_prepare<IT, TT, ST>(pic, (TT*)d_ref_t, (const TT*)d_hx, (const TT*)d_hy, sums,
                                                    d_begins, src, slice, slice, nb, nb, compute);
+                        launch_slice_stats<TT>((const TT*)d_ref_t, nullptr, slice.area(), nb, nb, (double*)d_rn2,
+                                               compute);
+                    }
                 } else {
                     if (!std::is_same<IT, uint16_t>::value)
                         launch_sum<IT, ST>(pic, sums, d_begins, src, slice, nb, nb, compute);
@@ -427,12 +451,21 @@ struct emida_offset_s {
             const size_t half = 2;  // bf16 / fp16
             d_ref = alloc((size_t)nb * slice.area() * half);
             if (mfma_split(mfma_mode)) d_ref_lo = alloc((size_t)nb * slice.area() * half);
-            if (mfma_refine) d_ref_t = alloc((size_t)nb * slice.area() * t_bytes());
+            if (mfma_refine) {
+                d_ref_t = alloc((size_t)nb * slice.area() * t_bytes());
+                d_rn2 = alloc((size_t)nb * sizeof(double));
+            }
             if (!resident_only) {
-                if (mfma_refine) d_slices = alloc(scratch_total * slice.area() * t_bytes());
+                if (mfma_refine) {
+                    d_slices = alloc(scratch_total * slice.area() * t_bytes());
+                    d_cnt = alloc(scratch_total * cross.y * sizeof(uint32_t));
+                    d_ent = alloc(scratch_total * cross.y * kListCap * sizeof(cand_entry));
+                    d_pn2 = alloc(scratch_total * 2 * sizeof(double));  // norms, then zero-shift products
+                } else {
+                    d_corr = alloc(scratch_total * cross.area() * sizeof(float));
+                }
                 d_pic_hi = alloc(scratch_total * slice.area() * half);
                 if (mfma_split(mfma_mode)) d_pic_lo = alloc(scratch_total * slice.area() * half);
-                d_corr = alloc(scratch_total * cross.area() * sizeof(float));
                 d_smax = alloc(scratch_total * cross.y * sizeof(uint32_t));
             }
         } else if (fftg) {
@@ -539,13 +572,13 @@ struct emida_offset_s {
     // Sums and preparation of a batch in one kernel for uint16 patterns (uint32 sums);
     // false when the input type or the slice size needs the separate kernels.
     template <class IT, class ST, class TT, class HT>
-    bool fused_prepare(const IT* pics, ST* sums, TT* slices, HT* hi, HT* lo, uint32_t* zero, uint32_t total,
+    bool fused_prepare(const IT* pics, ST* sums, TT* slices, HT* hi, HT* lo, double* stats, uint32_t total,
                        hipStream_t st) {
         if constexpr (std::is_same<IT, uint16_t>::value && std::is_same<ST, uint32_t>::value)
             return launch_sum_prepare<TT, HT>(pics, sums, slices, (const TT*)d_hx, (const TT*)d_hy, hi, lo,
                                               (const float*)float_hx(), (const float*)float_hy(),
-                                              hi ? mfma_in_scale<HT>() : 1.0f, zero, cross.y, d_begins, src, slice,
-                                              nb, total, st);
+                                              hi ? mfma_in_scale<HT>() : 1.0f, stats, (const TT*)d_ref_t, d_begins,
+                                              src, slice, nb, total, st);
         else
             return false;
     }
@@ -580,33 +613,45 @@ struct emida_offset_s {
                     launch_peak<TT>((const TT*)sc.corr, (size_t)tn * cross.area(), cross.area(), cross, POLICY_BRUTE,
                                     fit, (data_index<TT>*)maxes, (TT*)neigh, offsets, coefs, tn, st);
                 } else if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA) {
+                    // with refinement the kernel publishes the candidates (screen_lists)
+                    // instead of the screen
+                    screen_lists lists{};
+                    if (mfma_refine) {
+                        lists.cnt = (uint32_t*)sc.cnt;
+                        lists.ent = (cand_entry*)sc.ent;
+                        lists.pn2 = (const double*)sc.pn2;
+                        lists.rn2 = (const double*)d_rn2;
+                        lists.z0 = (const double*)sc.pn2 + tn;
+                        mfma_refine_bounds(mfma_mode, slice, T == EMIDA_F64, lists.tol_rel, lists.tol_abs);
+                    }
                     with_HT(mfma_mode, [&](auto ht) {
                         using HT = typename decltype(ht)::type;
+                        double* pn2 = mfma_refine ? (double*)sc.pn2 : nullptr;
                         if (!fused_prepare<IT, ST, TT, HT>(pics, sums, mfma_refine ? (TT*)sc.slices : nullptr,
-                                                           (HT*)sc.pic_hi, (HT*)sc.pic_lo, (uint32_t*)sc.smax, tn,
-                                                           st)) {
+                                                           (HT*)sc.pic_hi, (HT*)sc.pic_lo, pn2, tn, st)) {
                             launch_sum<IT, ST>(pics, sums, d_begins, src, slice, nb, tn, st);
                             launch_prepare_split<IT, ST, HT>(pics, (HT*)sc.pic_hi, (HT*)sc.pic_lo,
                                                              (const float*)float_hx(), (const float*)float_hy(), sums,
-                                                             d_begins, src, slice, nb, tn, st, (uint32_t*)sc.smax,
-                                                             cross.y);
-                            if (mfma_refine)
+                                                             d_begins, src, slice, nb, tn, st);
+                            if (mfma_refine) {
                                 launch_prepare<IT, TT, ST>(pics, (TT*)sc.slices, (const TT*)d_hx, (const TT*)d_hy,
                                                            sums, d_begins, src, slice, slice, nb, tn, st);
+                                launch_slice_stats<TT>((const TT*)sc.slices, (const TT*)d_ref_t, slice.area(), nb,
+                                                       tn, pn2, st);
+                            }
                         }
                         if (timing) t0 = record_start(st);
                         launch_mfma_corr<HT>((const HT*)sc.pic_hi, (const HT*)sc.pic_lo, (const HT*)d_ref,
-                                             (const HT*)d_ref_lo, (float*)sc.corr, slice, cross, nb, n_pat, st,
-                                             (uint32_t*)sc.smax);
+                                             (const HT*)d_ref_lo, mfma_refine ? nullptr : (float*)sc.corr, slice,
+                                             cross, nb, n_pat, st, (uint32_t*)sc.smax,
+                                             mfma_refine ? &lists : nullptr);
                         if (timing) record_stop(st, t0);
                     });
                     if (mfma_refine) {
-                        double tol_rel, tol_abs;
-                        mfma_refine_bounds(mfma_mode, slice, T == EMIDA_F64, tol_rel, tol_abs);
-                        launch_refine<TT>((const float*)sc.corr, (size_t)tn * cross.area(), (const TT*)sc.slices,
-                                          (const TT*)d_ref_t, slice, cross, nb, tol_rel, tol_abs, fit,
+                        launch_refine<TT>(nullptr, (size_t)tn * cross.area(), (const TT*)sc.slices,
+                                          (const TT*)d_ref_t, slice, cross, nb, lists.tol_rel, lists.tol_abs, fit,
                                           (data_index<TT>*)maxes, (TT*)neigh, offsets, coefs, tn, st,
-                                          (const uint32_t*)sc.smax);
+                                          (const uint32_t*)sc.smax, &lists);
                     } else {
                         launch_peak<float>((const float*)sc.corr, (size_t)tn * cross.area(), cross.area(), cross,
                                            POLICY_BRUTE, fit, (data_index<float>*)maxes, (float*)neigh, offsets,

@@ -447,13 +447,14 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
                                                     int fit, const double* __restrict__ lsq,
                                                     data_index<T>* __restrict__ maxes, T* __restrict__ neigh,
                                                     double* __restrict__ offsets, double* __restrict__ coefs,
-                                                    uint32_t total, const uint32_t* __restrict__ smax) {
+                                                    uint32_t total, const uint32_t* __restrict__ smax,
+                                                    screen_lists lists) {
     __shared__ float svf[NT / kWave];
     __shared__ uint32_t si[NT / kWave];
     __shared__ double sd[NT / kWave];
     __shared__ T sv[NT / kWave];
     __shared__ uint32_t cand[kRefineCap];
-    __shared__ uint32_t ncand, nrow;
+    __shared__ uint32_t ncand, nrow, overflow;
     __shared__ uint16_t rlist[kRowList];  // rows of the screen that can hold candidates
     __shared__ T nbv[81];
     __shared__ double cf[6];
@@ -466,7 +467,7 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
     const int tid = threadIdx.x;
     const uint32_t area = cross.area();
     const size_t a = (size_t)s.x * s.y;
-    const float* sc = screen + (size_t)sn * area;
+    const float* sc = screen ? screen + (size_t)sn * area : nullptr;  // none with candidate lists
     const T* ps = p + (size_t)sn * a;
     const T* rs = r + (size_t)(sn % nb) * a;
 
@@ -496,7 +497,11 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
         block_argmax<float, NT>(m, mi, svf, si);
     }
 
-    // ||p||, ||r|| in double
+    // ||p||, ||r|| in double (given with the candidate lists)
+    double tau;
+    if (lists.cnt) {
+        tau = refine_tau(lists, sn, sn % nb);
+    } else {
     double pp = 0, rr = 0;
     for (size_t i0 = tid; i0 < a; i0 += NT * U) {
         double u[U], w[U];
@@ -525,11 +530,12 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
     rr = 0;
     for (int k = 0; k < NT / kWave; ++k) rr += sd[k];
     const double np = sqrt(pp), nr = sqrt(rr);
-    const double tau = tol_rel * np * nr + tol_abs * (np + nr);
+    tau = tol_rel * np * nr + tol_abs * (np + nr);
+    }
 
     // candidates.  Given the row maxima, only rows whose maximum reaches the threshold
     // (or equals the slice maximum) can hold one, and only those rows are read.
-    if (tid == 0) ncand = nrow = 0;
+    if (tid == 0) ncand = nrow = overflow = 0;
     __syncthreads();
     const double thr = (double)m - tau;
     uint32_t mil = 0xffffffffu;  // lowest index holding m (given maximum)
@@ -540,7 +546,31 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
         }
         if (smax && idx < area && v == m && idx < mil) mil = idx;
     };
-    if (smax && cross.y <= (uint32_t)kRowList) {
+    if (lists.cnt) {
+        // the rows that can hold a candidate, then the positions they published (a row
+        // with more than kListCap of them makes the slice sweep)
+        for (uint32_t r = tid; r < cross.y; r += NT) {
+            const uint32_t code = smax[(size_t)sn * cross.y + r];
+            if (code && ((double)screen_max_value(code) >= thr || code == mcode)) {
+                const uint32_t k = atomicAdd(&nrow, 1u);
+                if (k < (uint32_t)kRowList) rlist[k] = r;
+                else overflow = 1;  // more rows than the list holds: sweep
+            }
+        }
+        __syncthreads();
+        const uint32_t n = min(nrow, (uint32_t)kRowList) * (uint32_t)kListCap;
+        for (uint32_t e = tid; e < n; e += NT) {
+            const uint32_t row = rlist[e / kListCap], k = e % kListCap;
+            const size_t rr = (size_t)sn * cross.y + row;
+            const uint32_t c = lists.cnt[rr];
+            if (c > (uint32_t)kListCap) {
+                if (k == 0) overflow = 1;
+            } else if (k < c) {
+                const cand_entry en = lists.ent[rr * kListCap + k];
+                take(row * cross.x + en.x, en.v);
+            }
+        }
+    } else if (smax && cross.y <= (uint32_t)kRowList) {
         for (uint32_t r = tid; r < cross.y; r += NT) {
             const uint32_t code = smax[(size_t)sn * cross.y + r];
             if (code && ((double)screen_max_value(code) >= thr || code == mcode)) rlist[atomicAdd(&nrow, 1u)] = r;
@@ -570,7 +600,7 @@ __global__ __launch_bounds__(NT) void refine_kernel(const float* __restrict__ sc
     }
     if (smax) mi = block_min_u32<NT>(mil, si);
     __syncthreads();
-    const bool sweep = ncand > (uint32_t)kRefineCap || mi == 0xffffffffu;
+    const bool sweep = ncand > (uint32_t)kRefineCap || mi == 0xffffffffu || overflow;
     const uint32_t nc = sweep ? 0u : ncand;
     // exact values; arg-max with the lowest index on ties.  Candidates whose bounding
     // box fits one box evaluation take the staged path, others one chain each.
@@ -713,22 +743,27 @@ void launch_subpixel(const T* neigh, int s, uint32_t n, const data_index<T>* max
 template <class T>
 void launch_refine(const float* screen, size_t corr_len, const T* p, const T* r, size2 s, size2 cross, uint32_t nb,
                    double tol_rel, double tol_abs, int fit, data_index<T>* maxes, T* neigh, double* offsets,
-                   double* coefs, uint32_t total, hipStream_t st, const uint32_t* screen_max) {
+                   double* coefs, uint32_t total, hipStream_t st, const uint32_t* screen_max,
+                   const screen_lists* lists) {
     if (total == 0) return;
     if (s.x > 2 * (uint32_t)kWave) throw arg_error("exact refinement needs slices at most 128 wide");
+    if (lists && (!screen_max || !lists->cnt || !lists->ent || !lists->pn2 || !lists->rn2))
+        throw arg_error("exact refinement: candidate lists need the row maxima, counts, entries and norms");
+    if (!lists && !screen) throw arg_error("exact refinement: no screen");
+    screen_lists none{};
     const double* lsq = lstsq_device_table() + lstsq_offset(fit);
     const size_t lds = (size_t)(kBoxY + kBoxRows - 1) * box_stride((int)s.x + kBoxW - 1) * sizeof(T);
     refine_kernel<T, kRefineNT><<<total, kRefineNT, lds, st>>>(screen, corr_len, p, r, s, cross, nb, tol_rel, tol_abs,
                                                           fit, lsq, maxes, neigh, offsets, coefs, total,
-                                                          screen_max);
+                                                          screen_max, lists ? *lists : none);
     EMIDA_LAUNCHED("refine_kernel");
 }
 template void launch_refine<float>(const float*, size_t, const float*, const float*, size2, size2, uint32_t, double,
                                    double, int, data_index<float>*, float*, double*, double*, uint32_t,
-                                   hipStream_t, const uint32_t*);
+                                   hipStream_t, const uint32_t*, const screen_lists*);
 template void launch_refine<double>(const float*, size_t, const double*, const double*, size2, size2, uint32_t,
                                     double, double, int, data_index<double>*, double*, double*, double*, uint32_t,
-                                    hipStream_t, const uint32_t*);
+                                    hipStream_t, const uint32_t*, const screen_lists*);
 
 #define INST_PEAK(T)                                                                                          \
     template void launch_peak<T>(const T*, size_t, uint32_t, size2, int, int, data_index<T>*, T*, double*,     \

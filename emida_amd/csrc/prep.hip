@@ -86,15 +86,19 @@ __global__ __launch_bounds__(256) void sum_prepare_kernel(const uint16_t* __rest
                                                           const T* __restrict__ hx, const T* __restrict__ hy,
                                                           HT* __restrict__ hi, HT* __restrict__ lo,
                                                           const float* __restrict__ hxf, const float* __restrict__ hyf,
-                                                          float in_scale, uint32_t* __restrict__ zero, uint32_t zrow,
+                                                          float in_scale, double* __restrict__ stats,
+                                                          const T* __restrict__ ref,
                                                           const size2* __restrict__ begins, size2 src, size2 slice,
                                                           uint32_t nb, uint32_t total) {
 #pragma clang fp contract(off)
     constexpr uint32_t NT = 256;
     extern __shared__ __align__(16) uint16_t spx[];  // the slice's pixels, row-major
     __shared__ uint32_t part[NT / kWave];
-    const uint32_t sn = blockIdx.x;
-    if (sn >= total) return;
+    __shared__ double partd[2][NT / kWave];
+    if (blockIdx.x >= total) return;
+    // ROI-major order: consecutive workgroups share a ROI (and its reference slice)
+    const uint32_t npat = total / nb;
+    const uint32_t sn = (blockIdx.x % npat) * nb + blockIdx.x / npat;
     const uint32_t area = slice.area(), sx = slice.x, tid = threadIdx.x;
     const size2 b0 = begins[sn % nb];
     const uint16_t* pic = pics + (size_t)(sn / nb) * src.area() + (size_t)b0.y * src.x + b0.x;
@@ -119,8 +123,6 @@ __global__ __launch_bounds__(256) void sum_prepare_kernel(const uint16_t* __rest
     uint32_t sum = 0;
     for (uint32_t k = 0; k < NT / kWave; ++k) sum += part[k];
     if (tid == 0 && sums) sums[sn] = sum;
-    if (zero)
-        for (uint32_t r = tid; r < zrow; r += NT) zero[(size_t)sn * zrow + r] = 0u;
     // rows of a multiple of 4 pixels: 4 consecutive pixels of one row per thread and
     // step (one 8-byte LDS read, vector table loads and stores); others one pixel
     const bool quad = sx % 4 == 0;
@@ -128,6 +130,8 @@ __global__ __launch_bounds__(256) void sum_prepare_kernel(const uint16_t* __rest
     if (slices) {
         T* out = slices + (size_t)sn * area;
         const T mean = (T)sum / (T)area;
+        double nn = 0, zz = 0;  // sums of squares and of products with the reference
+        const T* rs = stats ? ref + (size_t)(sn % nb) * area : nullptr;
         for (uint32_t e = tid * step; e < area; e += NT * step) {
             const uint32_t x = e % sx, y = e / sx;
             if (quad) {
@@ -142,11 +146,38 @@ __global__ __launch_bounds__(256) void sum_prepare_kernel(const uint16_t* __rest
                 v.z = ((T)q.z - mean) * wx.z * wy;
                 v.w = ((T)q.w - mean) * wx.w * wy;
                 *reinterpret_cast<t4*>(out + e) = v;
+                if (rs) {
+                    const t4 r = *reinterpret_cast<const t4*>(rs + e);
+                    nn += (double)v.x * (double)v.x + (double)v.y * (double)v.y + (double)v.z * (double)v.z +
+                          (double)v.w * (double)v.w;
+                    zz += (double)v.x * (double)r.x + (double)v.y * (double)r.y + (double)v.z * (double)r.z +
+                          (double)v.w * (double)r.w;
+                }
             } else {
                 T v = (T)spx[e];
                 v -= mean;
                 v = v * hx[x] * hy[y];
                 out[e] = v;
+                if (rs) {
+                    nn += (double)v * (double)v;
+                    zz += (double)v * (double)rs[e];
+                }
+            }
+        }
+        if (stats) {
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+                nn += __shfl_xor(nn, off);
+                zz += __shfl_xor(zz, off);
+            }
+            if (tid % kWave == 0) {
+                partd[0][tid / kWave] = nn;
+                partd[1][tid / kWave] = zz;
+            }
+            __syncthreads();
+            if (tid < 2) {
+                double t = 0;
+                for (uint32_t k = 0; k < NT / kWave; ++k) t += partd[tid][k];
+                stats[tid * (size_t)total + sn] = t;
             }
         }
     }
@@ -195,21 +226,79 @@ __global__ __launch_bounds__(256) void sum_prepare_kernel(const uint16_t* __rest
 
 template <class T, class HT>
 bool launch_sum_prepare(const uint16_t* pics, uint32_t* sums, T* slices, const T* hx, const T* hy, HT* hi, HT* lo,
-                        const float* hxf, const float* hyf, float in_scale, uint32_t* zero, uint32_t zrow,
+                        const float* hxf, const float* hyf, float in_scale, double* stats, const T* ref,
                         const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total, hipStream_t st) {
+    if (stats && (!slices || !ref)) throw arg_error("sum_prepare: statistics need the prepared and reference slices");
+    if (nb == 0 || total % nb) throw arg_error("sum_prepare: slices must be whole sets of ROIs");
     const size_t lds = (size_t)slice.area() * sizeof(uint16_t);
     if (lds > kSumPrepareMaxLds) return false;
     // the four-pixel steps read window tables and write outputs as vectors
     auto aligned = [](const void* q, size_t a) { return q == nullptr || (uintptr_t)q % a == 0; };
-    if (!aligned(slices, 4 * sizeof(T)) || !aligned(hx, 4 * sizeof(T)) || !aligned(hi, 4 * sizeof(HT)) ||
+    if (!aligned(slices, 4 * sizeof(T)) || !aligned(hx, 4 * sizeof(T)) || !aligned(ref, 4 * sizeof(T)) ||
+        !aligned(hi, 4 * sizeof(HT)) ||
         !aligned(lo, 4 * sizeof(HT)) || !aligned(hxf, 4 * sizeof(float)))
         return false;
     if (total == 0) return true;
-    sum_prepare_kernel<T, HT><<<total, 256, lds, st>>>(pics, sums, slices, hx, hy, hi, lo, hxf, hyf, in_scale, zero,
-                                                       zrow, begins, src, slice, nb, total);
+    sum_prepare_kernel<T, HT><<<total, 256, lds, st>>>(pics, sums, slices, hx, hy, hi, lo, hxf, hyf, in_scale, stats,
+                                                       ref, begins, src, slice, nb, total);
     EMIDA_LAUNCHED("sum_prepare_kernel");
     return true;
 }
+
+// Sums of squares and (with ref) of products with the ROI's reference slice, one
+// slice per workgroup, in double.
+template <class T>
+__global__ __launch_bounds__(256) void slice_stats_kernel(const T* __restrict__ slices, const T* __restrict__ ref,
+                                                          uint32_t area, uint32_t nb, uint32_t n,
+                                                          double* __restrict__ out) {
+    constexpr uint32_t NT = 256, U = 8;  // loads in flight per thread
+    __shared__ double part[2][NT / kWave];
+    const uint32_t sn = blockIdx.x, tid = threadIdx.x;
+    if (sn >= n) return;
+    const T* q = slices + (size_t)sn * area;
+    const T* rs = ref ? ref + (size_t)(sn % nb) * area : nullptr;
+    double nn = 0, zz = 0;
+    for (uint32_t e0 = tid; e0 < area; e0 += NT * U) {
+        double v[U], w[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k) {
+            const bool in = e0 + k * NT < area;
+            v[k] = in ? (double)q[e0 + k * NT] : 0.0;
+            w[k] = in && rs ? (double)rs[e0 + k * NT] : 0.0;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k) {
+            nn += v[k] * v[k];
+            zz += v[k] * w[k];
+        }
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        nn += __shfl_xor(nn, off);
+        zz += __shfl_xor(zz, off);
+    }
+    if (tid % kWave == 0) {
+        part[0][tid / kWave] = nn;
+        part[1][tid / kWave] = zz;
+    }
+    __syncthreads();
+    if (tid < (ref ? 2u : 1u)) {
+        double t = 0;
+        for (uint32_t k = 0; k < NT / kWave; ++k) t += part[tid][k];
+        out[tid * (size_t)n + sn] = t;
+    }
+}
+
+template <class T>
+void launch_slice_stats(const T* slices, const T* ref, uint32_t area, uint32_t nb, uint32_t n, double* out,
+                        hipStream_t st) {
+    if (n == 0) return;
+    slice_stats_kernel<T><<<n, 256, 0, st>>>(slices, ref, area, nb, n, out);
+    EMIDA_LAUNCHED("slice_stats_kernel");
+}
+template void launch_slice_stats<float>(const float*, const float*, uint32_t, uint32_t, uint32_t, double*,
+                                        hipStream_t);
+template void launch_slice_stats<double>(const double*, const double*, uint32_t, uint32_t, uint32_t, double*,
+                                         hipStream_t);
 
 template <class IN, class S>
 void launch_sum(const IN* pics, S* sums, const size2* begins, size2 src, size2 slice, uint32_t nb, uint32_t total,
@@ -232,7 +321,7 @@ void launch_prepare(const IN* pics, T* slices, const T* hx, const T* hy, const S
 
 #define INST_SUM_PREP(T, HT)                                                                              \
     template bool launch_sum_prepare<T, HT>(const uint16_t*, uint32_t*, T*, const T*, const T*, HT*, HT*,   \
-                                            const float*, const float*, float, uint32_t*, uint32_t,            \
+                                            const float*, const float*, float, double*, const T*,              \
                                             const size2*, size2, size2, uint32_t, uint32_t, hipStream_t);
 INST_SUM_PREP(double, _Float16)
 INST_SUM_PREP(double, __bf16)

@@ -183,7 +183,8 @@ def test_refined_equals_brute_policy_edge_peaks(cross):
 @pytest.mark.parametrize("mode", [emida_amd.MFMA_F16_SPLIT, emida_amd.MFMA_BF16])
 def test_refined_equals_brute_policy_tall_window(mode):
     # 64 x 144 slices with the full 127 x 287 window: more screen rows than the
-    # refinement's row list holds, so every row of the screen is read for candidates
+    # refinement's row list holds (only rows that can hold a candidate are listed; a
+    # slice with more of them than the list holds is swept)
     ps = synth.PatternSet("mfma-tall", 256, 2, t_range=2.0)
     begins = [(16, 8), (150, 100), (96, 40)]
     got = _run(ps, begins, (64, 144), (127, 287), 5, 2, T=np.float64, mode=mode)

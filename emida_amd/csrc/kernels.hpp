@@ -183,10 +183,21 @@ void launch_fftg_std_corr(const T* pic, const cplx<T>* spectra, T* res, size2 si
 // (optional, [pattern * nb + roi][cross.y]): the largest value of each row of each
 // slice (NaN ignored) as an ordered code, 0 for a row without one.  lists (optional,
 // with screen_max): the refinement's candidates per row (screen_lists).
+// src (optional): uint16 patterns prepared while they are staged, in place of phi/plo
+// (which are then unused; the split mode follows rlo), with prepare_split_kernel's
+// arithmetic, so the screen is the same.
+struct mfma_src {
+    const uint16_t* pics;   // [pattern][src.y][src.x]
+    const uint32_t* sums;   // [pattern * nb + roi]: slice sums
+    const float* hx;        // float Hann windows
+    const float* hy;
+    const size2* begins;    // ROI corners
+    size2 src;              // pattern size
+};
 template <class HT>
 void launch_mfma_corr(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
                       uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* screen_max = nullptr,
-                      const screen_lists* lists = nullptr);
+                      const screen_lists* lists = nullptr, const mfma_src* src = nullptr);
 void check_mfma_shape(size2 s, size2 cross);
 template <class IN, class S, class HT>
 void launch_prepare_split(const IN* pics, HT* hi, HT* lo, const float* hx, const float* hy, const S* sums,

@@ -117,14 +117,35 @@ struct mfma_plan {
 // hi/lo: prepared pattern slices [slice][y][x] (slice = pattern * nb + roi);
 // rhi/rlo: prepared reference slices [roi][y][x]; corr: [slice][cross.y][cross.x].
 // n_wg = nb * ceil(npat / 32) * ceil(cross.y / ROWS) workgroups (1-D grid).
-template <class HT, bool SPLIT, int KMAX, int WX>
+// The 8 prepared values of a row chunk from its uint16 pixels, prepare_split_kernel's
+// arithmetic: ((px - mean) * wx) * wy * in, then hi = round(v), lo = round(v - hi).
+template <class HT, bool SPLIT, class V>
+__device__ inline void prep_chunk(const uint32_t (&u)[4], float mean, const float (&wx)[8], float wy, V& h, V& l) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float v = (float)((u[k / 2] >> (16 * (k & 1))) & 0xffffu);
+        v -= mean;
+        v = v * wx[k];
+        v = v * wy;
+        v *= mfma_vec<HT>::in;
+        const HT hh = (HT)v;
+        h[k] = hh;
+        if constexpr (SPLIT) l[k] = (HT)(v - (float)hh);
+    }
+}
+
+// FP: the pattern rows are read as uint16 pixels and prepared while they are staged
+// (phi/plo unused); otherwise phi/plo hold the prepared halves.
+template <class HT, bool SPLIT, int KMAX, int WX, bool FP>
 __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restrict__ phi,
                                                                 const HT* __restrict__ plo,
                                                                 const HT* __restrict__ rhi,
                                                                 const HT* __restrict__ rlo,
                                                                 float* __restrict__ corr, size2 s, size2 cross,
                                                                 uint32_t nb, uint32_t npat, uint32_t n_wg,
-                                                                uint32_t* __restrict__ smax, screen_lists lists) {
+                                                                uint32_t* __restrict__ smax, screen_lists lists,
+                                                                mfma_src src) {
     using P = mfma_plan<HT, SPLIT, KMAX, WX>;
     using V = typename mfma_vec<HT>::type;
     constexpr int NT = kMfmaNT, TS = kMfmaTS, ROWS = P::ROWS;
@@ -179,21 +200,69 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
     // ---- staging through registers: global loads for step y + 1 are issued before the
     // matrix-core work of step y and written to LDS at the start of step y + 1
     V pfh[P::PF], pfl[P::PF];
-    auto load_prow = [&](int Y) {
+    // FP: each thread's chunks (pattern m, columns 8c..8c+7) are the same in every row:
+    // their pattern row pointer, mean and window values are set once; the pixels of
+    // the next row (8 uint16 per chunk, packed in pairs) and its window value are
+    // loaded ahead like the halves
+    const uint16_t* pbase[P::PF];
+    float pmean[P::PF], pwx[P::PF][8];
+    bool pval[P::PF];
+    uint32_t pfu[P::PF][4];
+    float pwy = 0.f;
+    if constexpr (FP) {
+        const size2 b0 = src.begins[roi];
 #pragma unroll
         for (int i = 0; i < P::PF; ++i) {
-            const int e = tid + NT * i;
-            V vh{}, vl{};
-            if (e < kMfmaM * CH && Y >= 0 && Y < SY) {
-                const int m = e / CH, c = e % CH;
-                if (p0 + m < npat) {
-                    const size_t g = ((size_t)(p0 + m) * nb + roi) * slice_elems + (size_t)Y * S + 8 * c;
-                    vh = *reinterpret_cast<const V*>(phi + g);
-                    if constexpr (SPLIT) vl = *reinterpret_cast<const V*>(plo + g);
+            const int e = tid + NT * i, m = e / CH, c = e % CH;
+            pval[i] = e < kMfmaM * CH && p0 + m < npat;
+            const uint32_t p = pval[i] ? p0 + m : p0;  // p0 < npat: a valid slice
+            const int cc = pval[i] ? c : 0;
+            pbase[i] = src.pics + (size_t)p * src.src.area() + (size_t)b0.y * src.src.x + b0.x + 8 * cc;
+            pmean[i] = (float)src.sums[(size_t)p * nb + roi] / (float)(S * SY);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) pwx[i][k] = src.hx[8 * cc + k];
+        }
+    }
+    auto load_prow = [&](int Y) {
+        if constexpr (FP) {
+            if (Y >= 0 && Y < SY) {
+                pwy = src.hy[Y];
+#pragma unroll
+                for (int i = 0; i < P::PF; ++i) {
+                    const uint16_t* q = pbase[i] + (size_t)Y * src.src.x;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) pfu[i][k] = (uint32_t)q[2 * k] | ((uint32_t)q[2 * k + 1] << 16);
                 }
             }
-            pfh[i] = vh;
-            pfl[i] = vl;
+        } else {
+#pragma unroll
+            for (int i = 0; i < P::PF; ++i) {
+                const int e = tid + NT * i;
+                V vh{}, vl{};
+                if (e < kMfmaM * CH && Y >= 0 && Y < SY) {
+                    const int m = e / CH, c = e % CH;
+                    if (p0 + m < npat) {
+                        const size_t g = ((size_t)(p0 + m) * nb + roi) * slice_elems + (size_t)Y * S + 8 * c;
+                        vh = *reinterpret_cast<const V*>(phi + g);
+                        if constexpr (SPLIT) vl = *reinterpret_cast<const V*>(plo + g);
+                    }
+                }
+                pfh[i] = vh;
+                pfl[i] = vl;
+            }
+        }
+    };
+    // FP: the loaded pixels into halves (a separate step, placed after the matrix-core
+    // work of a step rather than between the barrier and that work)
+    auto cvt_prow = [&]() {
+        if constexpr (FP) {
+#pragma unroll
+            for (int i = 0; i < P::PF; ++i) {
+                V h{}, l{};
+                if (pval[i]) prep_chunk<HT, SPLIT, V>(pfu[i], pmean[i], pwx[i], pwy, h, l);
+                pfh[i] = h;
+                pfl[i] = l;
+            }
         }
     };
     auto store_prow = [&](int Y) {
@@ -235,11 +304,13 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
     const int y_lo = max(0, -(sy0 + ROWS - 1)), y_hi = min(SY, SY - sy0);
     for (int Y = y_lo + sy0; Y < y_lo + sy0 + ROWS - 1; ++Y) {
         load_prow(Y);
+        cvt_prow();
         store_prow(Y);
     }
     if (y_lo < y_hi) {
         load_prow(y_lo + sy0 + ROWS - 1);
         load_rrow(y_lo);
+        cvt_prow();
     }
     const int m_a = lane % 32, half = lane / 32;
     for (int y = y_lo; y < y_hi; ++y) {
@@ -290,6 +361,9 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
                 }
             }
         }
+        // the next row's halves while this step's matrix-core work drains (converting
+        // them halfway through the step costs 4 spilled registers at 128 columns)
+        if (y + 1 < y_hi) cvt_prow();
         __syncthreads();
     }
     // ---- epilogue.  The workgroup holds complete screen rows (every x-shift of its
@@ -425,13 +499,14 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
 }
 
 // --------------------------------------------------------------- launchers
-template <class HT, bool SPLIT, int KMAX, int WX>
+template <class HT, bool SPLIT, int KMAX, int WX, bool FP>
 void launch_mfma_corr_g(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
-                        uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* smax, const screen_lists& lists) {
+                        uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* smax, const screen_lists& lists,
+                        const mfma_src& src) {
     using P = mfma_plan<HT, SPLIT, KMAX, WX>;
     static_assert(P::lds_bytes <= 80 * 1024, "two workgroups per CU");
     static_assert((size_t)P::ROWS * kMfmaM * 16 <= P::lds_bytes, "epilogue row tables exceed the LDS");
-    auto kern = mfma_corr_kernel<HT, SPLIT, KMAX, WX>;
+    auto kern = mfma_corr_kernel<HT, SPLIT, KMAX, WX, FP>;
     static const hipError_t attr =
         hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P::lds_bytes);
     EMIDA_HIP(attr);
@@ -439,20 +514,21 @@ void launch_mfma_corr_g(const HT* phi, const HT* plo, const HT* rhi, const HT* r
     const uint64_t n_wg = (uint64_t)nb * div_up(npat, (uint32_t)kMfmaM) * div_up(cross.y, (uint32_t)P::ROWS);
     if (n_wg > (1u << 24)) throw unsupported_error("matrix-core brute policy: too many workgroups in one launch");
     kern<<<(uint32_t)n_wg, kMfmaNT, P::lds_bytes, st>>>(phi, plo, rhi, rlo, corr, s, cross, nb, npat,
-                                                        (uint32_t)n_wg, smax, lists);
+                                                        (uint32_t)n_wg, smax, lists, src);
     EMIDA_LAUNCHED("mfma_corr_kernel");
 }
 
 // Row width class of a slice: 32, 64 or 128 columns (windows up to 64, 128, 256 wide).
-template <class HT, bool SPLIT>
+template <class HT, bool SPLIT, bool FP>
 void launch_mfma_corr_t(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
-                        uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* smax, const screen_lists& lists) {
+                        uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* smax, const screen_lists& lists,
+                        const mfma_src& src) {
     if (s.x <= 32)
-        launch_mfma_corr_g<HT, SPLIT, 32, 1>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax, lists);
+        launch_mfma_corr_g<HT, SPLIT, 32, 1, FP>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax, lists, src);
     else if (s.x <= 64)
-        launch_mfma_corr_g<HT, SPLIT, 64, 2>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax, lists);
+        launch_mfma_corr_g<HT, SPLIT, 64, 2, FP>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax, lists, src);
     else
-        launch_mfma_corr_g<HT, SPLIT, 128, 4>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax, lists);
+        launch_mfma_corr_g<HT, SPLIT, 128, 4, FP>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax, lists, src);
 }
 
 void check_mfma_shape(size2 s, size2 cross) {
@@ -464,18 +540,34 @@ void check_mfma_shape(size2 s, size2 cross) {
 
 template <class HT>
 void launch_mfma_corr(const HT* phi, const HT* plo, const HT* rhi, const HT* rlo, float* corr, size2 s, size2 cross,
-                      uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* smax, const screen_lists* lists) {
+                      uint32_t nb, uint32_t npat, hipStream_t st, uint32_t* smax, const screen_lists* lists,
+                      const mfma_src* src) {
     check_mfma_shape(s, cross);
     screen_lists none{};
+    if (src && (!src->pics || !src->sums || !src->hx || !src->hy || !src->begins))
+        throw arg_error("matrix-core brute policy: pattern staging needs the patterns, sums, windows and corners");
+    if (!src && !phi) throw arg_error("matrix-core brute policy: no pattern slices");
     if (lists && (!smax || !lists->cnt || !lists->ent || !lists->pn2 || !lists->rn2 || !lists->z0))
         throw arg_error("matrix-core brute policy: candidate lists need the row maxima, counts, entries and norms");
     if (!corr && !smax) throw arg_error("matrix-core brute policy: no output");
     if (nb == 0 || npat == 0) return;
     const screen_lists& L = lists ? *lists : none;
-    if (plo)
-        launch_mfma_corr_t<HT, true>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax, L);
-    else
-        launch_mfma_corr_t<HT, false>(phi, nullptr, rhi, nullptr, corr, s, cross, nb, npat, st, smax, L);
+    const bool split = rlo != nullptr;  // the reference's lo halves exist exactly in split modes
+    if (src) {
+        if (split)
+            launch_mfma_corr_t<HT, true, true>(nullptr, nullptr, rhi, rlo, corr, s, cross, nb, npat, st, smax, L, *src);
+        else
+            launch_mfma_corr_t<HT, false, true>(nullptr, nullptr, rhi, nullptr, corr, s, cross, nb, npat, st, smax, L,
+                                                *src);
+    } else {
+        if (split != (plo != nullptr)) throw arg_error("matrix-core brute policy: lo halves of one operand only");
+        const mfma_src none_src{};
+        if (split)
+            launch_mfma_corr_t<HT, true, false>(phi, plo, rhi, rlo, corr, s, cross, nb, npat, st, smax, L, none_src);
+        else
+            launch_mfma_corr_t<HT, false, false>(phi, nullptr, rhi, nullptr, corr, s, cross, nb, npat, st, smax, L,
+                                                 none_src);
+    }
 }
 
 // |screen - T-precision brute value| <= tol_rel ||p|| ||r|| + tol_abs (||p|| + ||r||)
@@ -542,7 +634,7 @@ void launch_split(const float* v, HT* hi, HT* lo, size_t n, hipStream_t st) {
 
 #define INST_MFMA(HT)                                                                                          \
     template void launch_mfma_corr<HT>(const HT*, const HT*, const HT*, const HT*, float*, size2, size2, uint32_t, \
-                                       uint32_t, hipStream_t, uint32_t*, const screen_lists*);                 \
+                                       uint32_t, hipStream_t, uint32_t*, const screen_lists*, const mfma_src*); \
     template void launch_split<HT>(const float*, HT*, HT*, size_t, hipStream_t);                               \
     template void launch_prepare_split<uint16_t, uint32_t, HT>(const uint16_t*, HT*, HT*, const float*,        \
                                                                const float*, const uint32_t*, const size2*, size2, \

@@ -237,8 +237,10 @@ struct emida_offset_s {
             } else {
                 parts[2] = al256(tn * cross.area() * sizeof(float));  // the screen
             }
-            parts[3] = al256(tn * slice.area() * 2);
-            if (mfma_split(mfma_mode)) parts[4] = parts[3];
+            if (IN != EMIDA_U16) {  // uint16 patterns are prepared by the matrix-core kernel itself
+                parts[3] = al256(tn * slice.area() * 2);
+                if (mfma_split(mfma_mode)) parts[4] = parts[3];
+            }
             parts[5] = al256(tn * cross.y * sizeof(uint32_t));
         } else if (fftg) {
             parts[2] = al256(fftg_work_bytes(T == EMIDA_F64, slice, (uint32_t)tn, true));  // half spectra + corr
@@ -464,8 +466,10 @@ struct emida_offset_s {
                 } else {
                     d_corr = alloc(scratch_total * cross.area() * sizeof(float));
                 }
-                d_pic_hi = alloc(scratch_total * slice.area() * half);
-                if (mfma_split(mfma_mode)) d_pic_lo = alloc(scratch_total * slice.area() * half);
+                if (IN != EMIDA_U16) {
+                    d_pic_hi = alloc(scratch_total * slice.area() * half);
+                    if (mfma_split(mfma_mode)) d_pic_lo = alloc(scratch_total * slice.area() * half);
+                }
                 d_smax = alloc(scratch_total * cross.y * sizeof(uint32_t));
             }
         } else if (fftg) {
@@ -627,12 +631,19 @@ struct emida_offset_s {
                     with_HT(mfma_mode, [&](auto ht) {
                         using HT = typename decltype(ht)::type;
                         double* pn2 = mfma_refine ? (double*)sc.pn2 : nullptr;
-                        if (!fused_prepare<IT, ST, TT, HT>(pics, sums, mfma_refine ? (TT*)sc.slices : nullptr,
-                                                           (HT*)sc.pic_hi, (HT*)sc.pic_lo, pn2, tn, st)) {
+                        // uint16 patterns: the matrix-core kernel prepares the pattern rows as
+                        // it stages them, so only the sums (and, for the refinement, the T
+                        // slices and their statistics) are made here
+                        constexpr bool u16 = std::is_same<IT, uint16_t>::value && std::is_same<ST, uint32_t>::value;
+                        HT* phi = u16 ? nullptr : (HT*)sc.pic_hi;
+                        HT* plo = u16 ? nullptr : (HT*)sc.pic_lo;
+                        if (!fused_prepare<IT, ST, TT, HT>(pics, sums, mfma_refine ? (TT*)sc.slices : nullptr, phi, plo,
+                                                           pn2, tn, st)) {
                             launch_sum<IT, ST>(pics, sums, d_begins, src, slice, nb, tn, st);
-                            launch_prepare_split<IT, ST, HT>(pics, (HT*)sc.pic_hi, (HT*)sc.pic_lo,
-                                                             (const float*)float_hx(), (const float*)float_hy(), sums,
-                                                             d_begins, src, slice, nb, tn, st);
+                            if (!u16)
+                                launch_prepare_split<IT, ST, HT>(pics, phi, plo, (const float*)float_hx(),
+                                                                 (const float*)float_hy(), sums, d_begins, src, slice,
+                                                                 nb, tn, st);
                             if (mfma_refine) {
                                 launch_prepare<IT, TT, ST>(pics, (TT*)sc.slices, (const TT*)d_hx, (const TT*)d_hy,
                                                            sums, d_begins, src, slice, slice, nb, tn, st);
@@ -640,11 +651,12 @@ struct emida_offset_s {
                                                        tn, pn2, st);
                             }
                         }
+                        const mfma_src msrc{(const uint16_t*)pics, (const uint32_t*)sums, (const float*)float_hx(),
+                                            (const float*)float_hy(), d_begins, src};
                         if (timing) t0 = record_start(st);
-                        launch_mfma_corr<HT>((const HT*)sc.pic_hi, (const HT*)sc.pic_lo, (const HT*)d_ref,
-                                             (const HT*)d_ref_lo, mfma_refine ? nullptr : (float*)sc.corr, slice,
-                                             cross, nb, n_pat, st, (uint32_t*)sc.smax,
-                                             mfma_refine ? &lists : nullptr);
+                        launch_mfma_corr<HT>(phi, plo, (const HT*)d_ref, (const HT*)d_ref_lo,
+                                             mfma_refine ? nullptr : (float*)sc.corr, slice, cross, nb, n_pat, st,
+                                             (uint32_t*)sc.smax, mfma_refine ? &lists : nullptr, u16 ? &msrc : nullptr);
                         if (timing) record_stop(st, t0);
                     });
                     if (mfma_refine) {

@@ -77,15 +77,54 @@ struct fft_plan {
     static constexpr int WAVES_PER_SIMD = WAVES_PER_SIMD_RAW < 1 ? 1 : (WAVES_PER_SIMD_RAW > 4 ? 4 : WAVES_PER_SIMD_RAW);
 };
 
-// W_M^t for t < M, from double precision
-template <class T, int M>
-__device__ __forceinline__ void init_twiddles(cplx<T>* tw, int tid, int nt) {
-    for (int t = tid; t < M; t += nt) {
-        double s, c;
-        sincospi(2.0 * t / M, &s, &c);
-        tw[t] = cplx<T>{(T)c, (T)(-s)};
+static __device__ cplx<float> g_tw_f[2 * kTwMaxM - 4];
+static __device__ cplx<double> g_tw_d[2 * kTwMaxM - 4];
+
+template <class T>
+const cplx<T>* fft_twiddles() {
+    static std::mutex mu;
+    static std::vector<std::pair<const void*, const void*>> per_device;
+    int dev = 0;
+    EMIDA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)per_device.size() <= dev) per_device.resize(dev + 1, {nullptr, nullptr});
+    if (!per_device[dev].first) {
+        std::vector<cplx<float>> hf(2 * kTwMaxM - 4);
+        std::vector<cplx<double>> hd(2 * kTwMaxM - 4);
+        const long double pi = 3.141592653589793238462643383279502884L;
+        for (int m = 4; m <= kTwMaxM; m *= 2)
+            for (int t = 0; t < m; ++t) {
+                const long double a = 2 * pi * t / m;
+                long double c = cosl(a), sn = sinl(a);
+                if (4 * t % m == 0) {  // multiples of a quarter turn: exact
+                    c = (t == 0) ? 1 : (2 * t == m ? -1 : 0);
+                    sn = (4 * t == m) ? 1 : (4 * t == 3 * m ? -1 : 0);
+                }
+                hd[m - 4 + t] = cplx<double>{(double)c, (double)-sn};
+                hf[m - 4 + t] = cplx<float>{(float)c, (float)-sn};
+            }
+        EMIDA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_tw_f), hf.data(), hf.size() * sizeof(cplx<float>)));
+        EMIDA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_tw_d), hd.data(), hd.size() * sizeof(cplx<double>)));
+        void *pf = nullptr, *pd = nullptr;
+        EMIDA_HIP(hipGetSymbolAddress(&pf, HIP_SYMBOL(g_tw_f)));
+        EMIDA_HIP(hipGetSymbolAddress(&pd, HIP_SYMBOL(g_tw_d)));
+        if (!pf || !pd) throw hip_error("hipGetSymbolAddress returned no address for the twiddle tables");
+        per_device[dev] = {pf, pd};
     }
+    return (const cplx<T>*)(sizeof(T) == 8 ? per_device[dev].second : per_device[dev].first);
 }
+template const cplx<float>* fft_twiddles<float>();
+template const cplx<double>* fft_twiddles<double>();
+
+// The larger of two transform outputs, a NaN operand ignored.  The outputs are
+// arithmetic results, never signalling NaNs, so float takes v_max_f32 as is (IEEE
+// maxNum) without the quieting of both operands fmax adds.
+__device__ __forceinline__ float max_quiet(float a, float b) {
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double max_quiet(double a, double b) { return fmax(a, b); }
 
 // The fit's least-squares rows (6 x kk doubles, kk = F^2 <= 81) in LDS: the loads are
 // issued at the start (clamped addresses, no branch around them), stored once the
@@ -143,14 +182,14 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
         T wx[E], wy[E];
 #pragma unroll
         for (int it = 0; it < E; ++it) {
-            const int e = min(tid + it * NT, s * s - 1);  // clamped: masked below
+            const int e = PAD ? min(tid + it * NT, s * s - 1) : tid + it * NT;  // clamped: masked below
             px[it] = pic[(size_t)(e / s) * a.src.x + (e % s)];
             wx[it] = a.hx[e % s];
             wy[it] = a.hy[e / s];
         }
         double lv[kLsqPer];
         load_lsq<NT>(lv, a.lsq, kk, tid);
-        init_twiddles<T, M>(tw, tid, NT);
+        load_twiddles<T, M>(tw, a.twt, tid, NT);
         store_lsq<NT>(lsq_s, lv, kk, tid);
         T mean;
         if constexpr (std::is_same<IN, uint16_t>::value) {
@@ -178,7 +217,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
         for (int e = tid; e < s * s; e += NT) pre[e] = src[e];
         double lv[kLsqPer];
         load_lsq<NT>(lv, a.lsq, kk, tid);
-        init_twiddles<T, M>(tw, tid, NT);
+        load_twiddles<T, M>(tw, a.twt, tid, NT);
         store_lsq<NT>(lsq_s, lv, kk, tid);
     }
     __syncthreads();
@@ -341,8 +380,8 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
                 slot[xx] = x[n1];
                 if (xx <= s || xx > s + pad) {
                     const uint32_t rx = (uint32_t)(xx <= s ? xx : xx - pad);
-                    if (in_a) vmax = fmax(vmax, x[n1].re);
-                    if (in_b) vmax = fmax(vmax, x[n1].im);
+                    if (in_a) vmax = max_quiet(vmax, x[n1].re);
+                    if (in_b) vmax = max_quiet(vmax, x[n1].im);
                     if (cout) {
                         if (in_a) cout[ra + rx] = x[n1].re;
                         if (in_b) cout[rb + rx] = x[n1].im;
@@ -475,6 +514,7 @@ void launch_fft_ref(const IN* pics, const S* sums, const T* hx, const T* hy, con
     a.nb = nb;
     a.total = nb;
     a.spectra_out = spectra;
+    a.twt = fft_twiddles<T>();
     if (nb == 0) return;
     launch_fft_side<T, IN, S, true, false>(side, a, nb, st);
 }
@@ -500,6 +540,7 @@ void launch_fft_pipe(const IN* pics, const S* sums, const T* hx, const T* hy, co
     a.offsets = offsets;
     a.coefs = coefs;
     a.corr_out = corr_out;
+    a.twt = fft_twiddles<T>();
     if (total == 0) return;
     launch_fft_side<T, IN, S, true, true>(side, a, total, st);
 }
@@ -514,6 +555,7 @@ static void fft_corr_prepared(const T* pics, const T* ref, T* res, uint32_t side
     a.total = ref_slices;
     a.prepared = ref;
     a.spectra_out = spectra;
+    a.twt = fft_twiddles<T>();
     launch_fft_side<T, T, T, false, false>(side, a, ref_slices, st);
     fft_args<T, T, T> b{};
     b.nb = ref_slices;
@@ -523,6 +565,7 @@ static void fft_corr_prepared(const T* pics, const T* ref, T* res, uint32_t side
     b.fit = 3;
     b.lsq = lstsq_device_table();
     b.corr_out = res;
+    b.twt = a.twt;
     launch_fft_side<T, T, T, false, true>(side, b, b.total, st);
 }
 

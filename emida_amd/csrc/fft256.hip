@@ -53,14 +53,6 @@ static_assert(6 * 81 <= kNT, "one least-squares element per thread");
 static_assert(4 * 81 <= kNT, "four threads per neighbourhood value");
 constexpr size_t kLds = (size_t)(kSlots + kM) * sizeof(cplx<float>);
 
-__device__ __forceinline__ void init_twiddles(cplx<float>* tw, int tid, int nt) {
-    for (int t = tid; t < kM; t += nt) {
-        double s, c;
-        sincospi(2.0 * t / kM, &s, &c);
-        tw[t] = cplx<float>{(float)c, (float)(-s)};
-    }
-}
-
 // Packed inverse of output rows (2p, 2p + 1): frequency k of the row pair is
 // Ga[k] + i Gb[k] with G[k] = col[k] for k <= S and conj(col[M - k]) above (DC and
 // Nyquist imaginary parts dropped, as a C2R does).  Column c's values at the two rows
@@ -190,7 +182,7 @@ __global__ __launch_bounds__(kNT, 1) void fft256_kernel(fft_args<float, IN, SUM>
     // the fit's least-squares rows (6 x F^2 <= 486 doubles), staged in LDS for the end
     const int kk = PIPE ? a.fit * a.fit : 0;
     const double lsq_v = tid < 6 * kk ? a.lsq[tid] : 0.0;
-    init_twiddles(tw, tid, kNT);  // while the loads are in flight
+    load_twiddles<float, kM>(tw, a.twt, tid, kNT);  // while the loads are in flight
     // mean of the ROI (FROM_PATTERN): every pixel is one (pair, row, column) of one lane
     float mean = 0.f;
     if constexpr (FROM_PATTERN) {

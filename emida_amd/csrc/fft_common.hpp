@@ -190,6 +190,13 @@ struct fft_args {
     double* offsets;
     double* coefs;
     T* corr_out;             // optional (2s)^2 circular buffers in the reference layout, row-major
+    const cplx<T>* twt;      // twiddle tables (fft_twiddles)
 };
+
+// Copy of the M-entry table into LDS.
+template <class T, int M>
+__device__ __forceinline__ void load_twiddles(cplx<T>* tw, const cplx<T>* table, int tid, int nt) {
+    for (int t = tid; t < M; t += nt) tw[t] = table[M - 4 + t];
+}
 
 }  // namespace emida_amd

@@ -108,13 +108,10 @@ __device__ __forceinline__ cplx<T>* stockham(cplx<T>* a, cplx<T>* b, const cplx<
     return a;
 }
 
+// W_n^t for t < n from the twiddle tables (table of n at offset n - 4).
 template <class T>
-__device__ __forceinline__ void init_tw(cplx<T>* tw, int n, int tid, int nt) {
-    for (int t = tid; t < n; t += nt) {
-        double s, c;
-        sincospi(2.0 * t / n, &s, &c);
-        tw[t] = cplx<T>{(T)c, (T)(-s)};
-    }
+__device__ __forceinline__ void init_tw(cplx<T>* tw, const cplx<T>* table, int n, int tid, int nt) {
+    for (int t = tid; t < n; t += nt) tw[t] = table[n - 4 + t];
 }
 
 }  // namespace
@@ -142,6 +139,7 @@ struct fftg_args {
     T scale;                 // spectrum out: factor; product: factor on the product
     size2 out;               // output layout of the inverse rows kernel (2s.x x 2s.y)
     T* corr;                 // [slice - first][out.y][out.x]
+    const cplx<T>* twt;      // twiddle tables (fft_twiddles)
 };
 
 // ---------------------------------------------------------------- kernel 1: rows
@@ -156,7 +154,7 @@ __global__ __launch_bounds__(kGNT) void fftg_rows_kernel(fftg_args<T, IN, SUM> a
     C* A = tw + MX + g * 2 * MX;
     C* Bf = A + MX;
     const uint32_t sl = blockIdx.x, sn = a.first + sl;
-    init_tw<T>(tw, MX, tid, kGNT);
+    init_tw<T>(tw, a.twt, MX, tid, kGNT);
 
     const IN* pic = nullptr;
     const T* pre = nullptr;
@@ -237,7 +235,7 @@ __global__ __launch_bounds__(col_tile<T>(MY) * gsize(MY)) void fftg_cols_kernel(
     const int tid = threadIdx.x, g = tid / G, lane = tid % G;
     const uint32_t sl = blockIdx.y, sn = a.first + sl, roi = sn % a.nb;
     const int ncx = (int)a.mx / 2 + 1, kx0 = blockIdx.x * CB;
-    init_tw<T>(tw, MY, tid, NT);
+    init_tw<T>(tw, a.twt, MY, tid, NT);
     C* w = a.work + (size_t)sl * a.my * ncx;
     const int rows = MODE == COLS_SPECTRUM || MODE == COLS_PIPE ? (int)a.side.y : (int)a.my;
     // rows of the output the inverse rows kernel reads (the reference layout)
@@ -299,7 +297,7 @@ __global__ __launch_bounds__(kGNT) void fftg_rows_inv_kernel(fftg_args<T, T, T> 
     C* A = tw + MX + g * 2 * MX;
     C* Bf = A + MX;
     const uint32_t sl = blockIdx.x;
-    init_tw<T>(tw, MX, tid, kGNT);
+    init_tw<T>(tw, a.twt, MX, tid, kGNT);
     __syncthreads();
     const C* w = a.work + (size_t)sl * a.my * NCX;
     T* out = a.corr + (size_t)sl * a.out.area();
@@ -421,6 +419,7 @@ fftg_args<T, T, T> as_t(const fftg_args<T, IN, SUM>& a) {
     b.scale = a.scale;
     b.out = a.out;
     b.corr = a.corr;
+    b.twt = a.twt;
     return b;
 }
 
@@ -462,6 +461,7 @@ void launch_fftg_ref(const IN* pics, const S* sums, const T* hx, const T* hy, co
                      size2 side, uint32_t nb, cplx<T>* spectra, void* work, hipStream_t st) {
     if (nb == 0) return;
     fftg_args<T, IN, S> a{};
+    a.twt = fft_twiddles<T>();
     a.pics = pics;
     a.sums = sums;
     a.hx = hx;
@@ -505,6 +505,7 @@ void launch_fftg_pipe(const IN* pics, const S* sums, const T* hx, const T* hy, c
     T* corr_chunk = corr_out ? nullptr : (T*)(wb + chunk * fftg_slice_work_bytes(dbl, side));
     for (uint32_t f = 0; f < total; f += (uint32_t)chunk) {
         fftg_args<T, IN, S> a{};
+        a.twt = fft_twiddles<T>();
         a.pics = pics;
         a.sums = sums;
         a.hx = hx;
@@ -540,6 +541,7 @@ template <class T>
 void launch_fftg_corr_prepared(const T* pics, const T* ref, T* res, size2 side, uint32_t nb, uint32_t batch,
                                cplx<T>* spectra, void* work, size_t chunk, hipStream_t st) {
     fftg_args<T, T, T> a{};
+    a.twt = fft_twiddles<T>();
     a.prepared = ref;
     a.pitch = side.x;
     a.slice_elems = side.area();
@@ -586,6 +588,7 @@ template <class T>
 void launch_fftg_std_corr(const T* pic, const cplx<T>* spectra, T* res, size2 side, uint32_t nb, uint32_t total,
                           void* work, size_t chunk, hipStream_t st) {
     fftg_args<T, T, T> a{};
+    a.twt = fft_twiddles<T>();
     a.prepared = pic;
     a.pitch = 2 * side.x;
     a.slice_elems = 4 * side.area();
@@ -618,6 +621,7 @@ template <class T>
 void launch_fftg_std_r2c(const T* pic, uint32_t pitch, cplx<T>* spectra, size2 side, uint32_t n, void* work,
                          size_t chunk, hipStream_t st) {
     fftg_args<T, T, T> a{};
+    a.twt = fft_twiddles<T>();
     a.prepared = pic;
     a.pitch = pitch;
     a.slice_elems = pitch * 2 * side.y;

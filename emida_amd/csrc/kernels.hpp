@@ -111,6 +111,12 @@ void launch_refine(const float* screen, size_t corr_len, const T* p, const T* r,
 void mfma_refine_bounds(int mode, size2 s, bool t_double, double& tol_rel, double& tol_abs);
 
 // ---- fft.hip, fft256.hip
+// W_M^t = exp(-2 pi i t / M) for every power of two 4 <= M <= kTwMaxM, the table of M at
+// offset M - 4 (M entries), rounded from extended precision; a static device array per
+// GPU, filled on first use (no allocation).
+constexpr int kTwMaxM = 512;
+template <class T>
+const cplx<T>* fft_twiddles();
 // Largest padded FFT side of the single-kernel pipelines: 128 for double, 256 for
 // float (M = 256 in fft256.hip); fftg.hip takes every other shape up to kFftMaxSlice
 // per axis.

@@ -489,7 +489,8 @@ struct emida_offset_s {
             o.coefs = (double*)alloc(total * 6 * sizeof(double));
             EMIDA_HIP(hipEventCreateWithFlags(&o.copied, hipEventDisableTiming));
         }
-        (void)lstsq_device_table();
+        (void)lstsq_device_table();  // the device tables, before any batch (or graph capture)
+        (void)fft_twiddles<float>();
         if (temp) prepare_reference(temp);
         if (temp && policy == EMIDA_CROSS_POLICY_FFT && !resident_only)
             h_ref.assign((const unsigned char*)temp, (const unsigned char*)temp + (size_t)src.area() * in_bytes());

@@ -87,6 +87,11 @@ int lstsq_offset(int fit) {
     }
 }
 
+// The tables live in a static device array (each GPU holds its own copy): filled once
+// per device with a copy, never allocated, so entry points that must not allocate can
+// use them; host code uses the address hipGetSymbolAddress gives, never the name.
+static __device__ double g_lstsq[6 * (9 + 25 + 49 + 81)];
+
 const double* lstsq_device_table() {
     static std::mutex mu;
     static std::vector<double*> per_device;
@@ -97,10 +102,11 @@ const double* lstsq_device_table() {
     if (!per_device[dev]) {
         std::vector<double> host(6 * (9 + 25 + 49 + 81));
         for (int s = 3; s <= 9; s += 2) lstsq_host(s, host.data() + lstsq_offset(s));
-        double* d = nullptr;
-        EMIDA_HIP(hipMalloc(&d, host.size() * sizeof(double)));
-        EMIDA_HIP(hipMemcpy(d, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice));
-        per_device[dev] = d;
+        EMIDA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_lstsq), host.data(), host.size() * sizeof(double)));
+        void* d = nullptr;
+        EMIDA_HIP(hipGetSymbolAddress(&d, HIP_SYMBOL(g_lstsq)));
+        if (!d) throw hip_error("hipGetSymbolAddress returned no address for the least-squares table");
+        per_device[dev] = (double*)d;
     }
     return per_device[dev];
 }

@@ -345,18 +345,16 @@ __device__ inline T exact_at(size_t g, size_t corr_len, const T* p, const T* r, 
 
 // Exact values of a box of shifts of one slice, all threads of the workgroup
 // together: thread t < bw * bh takes position (x0 + t % bw, y0 + t / bw) of the cross
-// buffer (shift = position - (cross - 1) / 2).  The lanes of a wave walk the union of
-// the box's overlaps in lockstep (y outer, x inner): chunks of kBoxY rows of the
-// pattern slice are staged in LDS with zeros wherever a shift leaves the slice (row
-// stride below), every lane takes the same x at each step,
-// and the reference value of that step is broadcast from the lane holding it (each
-// lane holds one or two elements of the current reference row, so slices are at most
-// 2 * kWave wide), so LDS serves one operand per product instead of two.  The terms
-// a lane adds outside its own overlap are exact zeros (fma(0, r, acc) == acc for
-// finite values, and the accumulator is never -0), so each value equals exact_corr's
-// bit for bit.  LDS: kBoxY + kBoxRows - 1 rows of box_stride(s.x + kBoxW - 1), which
-// also holds the box values afterwards; boxes of at most 14 rows keep it at 23 KB for
-// 128-wide double slices, so six workgroups share a CU (the register limit).
+// buffer (shift = position - (cross - 1) / 2).  The lanes of a wave walk the union of the
+// box's overlaps in lockstep (y outer, x inner): chunks of kBoxY rows of the pattern
+// slice are staged in LDS with zeros wherever a shift leaves the slice (row stride
+// below), every lane takes the same x at each step, and the reference value of that
+// step is the same for every lane, read through the scalar unit; LDS serves one
+// operand per fma.  The terms a lane adds outside its own overlap are exact zeros
+// (fma(0, r, acc) == acc for finite values, and the accumulator is never -0), so each
+// value equals exact_corr's bit for bit.  LDS: kBoxY + kBoxRows - 1 rows of
+// box_stride(s.x + kBoxW - 1), which also holds the box values afterwards; boxes of at
+// most 14 rows keep it at 23 KB for 128-wide double slices.
 constexpr int kBoxRows = 14;
 constexpr int kBoxW = 32;
 constexpr int kBoxY = 4;
@@ -366,16 +364,6 @@ static_assert(kSweepRows * kBoxW <= kRefineNT && kSweepRows <= kBoxRows, "sweep 
 // 32.  A box whose conflict-free stride does not fit it falls back to this stride, on
 // which consecutive box rows start 18 banks apart (at most 2-way conflicts).
 __host__ __device__ constexpr int box_stride(int sx) { return sx + ((9 - sx % 32) + 32) % 32; }
-
-// The value lane l of the wave holds (l wave-uniform), as a scalar operand.
-__device__ inline double bcast(double v, int l) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-__device__ inline float bcast(float v, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
 
 template <class T, int NT>
 __device__ inline void eval_box(const T* __restrict__ p, const T* __restrict__ r, size2 s, size2 cross, int x0,
@@ -398,12 +386,6 @@ __device__ inline void eval_box(const T* __restrict__ p, const T* __restrict__ r
     const bool wave_on = tid - tid % kWave < n;  // waves with no position skip the sums
     const int t = min(tid, n - 1), lx = t % bw, ly = t / bw;
     T acc = 0;
-    const int lane = tid % kWave, l0 = min(lane, S - 1), l1 = min(lane + kWave, S - 1);
-    T rn0 = 0, rn1 = 0;  // the next reference row, one element per lane (two for S > 64)
-    if (ylo < yhi) {
-        rn0 = r[(size_t)ylo * S + l0];
-        rn1 = r[(size_t)ylo * S + l1];
-    }
     for (int yc = ylo; yc < yhi; yc += kBoxY) {
         const int ny = min(kBoxY, yhi - yc);
         __syncthreads();
@@ -415,30 +397,22 @@ __device__ inline void eval_box(const T* __restrict__ p, const T* __restrict__ r
         if (!wave_on) continue;
         for (int k = 0; k < ny; ++k) {
             const T* pr = prow + (k + ly) * PS + lx - xlo;  // pr[x] = pattern (y + sy, x + sx)
-            const T r0 = rn0, r1 = rn1;
-            if (yc + k + 1 < yhi) {
-                rn0 = r[(size_t)(yc + k + 1) * S + l0];
-                rn1 = r[(size_t)(yc + k + 1) * S + l1];
-            }
+            // the reference row is the same for every lane: read through the scalar unit
+            // (before, a v_readlane pair per product broadcast it from the lane holding it)
+            const T* rr = r + (size_t)(yc + k) * S;
             int x = xlo;
-            const int xm = min(xhi, kWave);  // x < kWave from r0, the rest from r1
-            // the pattern operands of 8 products are loaded before their 8 dependent fmas
-            for (; x + 8 <= xm; x += 8) {
-                T u[8];
-#pragma unroll
-                for (int i = 0; i < 8; ++i) u[i] = pr[x + i];
-#pragma unroll
-                for (int i = 0; i < 8; ++i) acc = fma(u[i], bcast(r0, x + i), acc);
-            }
-            for (; x < xm; ++x) acc = fma(pr[x], bcast(r0, x), acc);
+            // the operands of 8 products are loaded before their 8 dependent fmas
             for (; x + 8 <= xhi; x += 8) {
-                T u[8];
+                T u[8], q[8];
 #pragma unroll
-                for (int i = 0; i < 8; ++i) u[i] = pr[x + i];
+                for (int i = 0; i < 8; ++i) {
+                    u[i] = pr[x + i];
+                    q[i] = rr[x + i];
+                }
 #pragma unroll
-                for (int i = 0; i < 8; ++i) acc = fma(u[i], bcast(r1, x + i - kWave), acc);
+                for (int i = 0; i < 8; ++i) acc = fma(u[i], q[i], acc);
             }
-            for (; x < xhi; ++x) acc = fma(pr[x], bcast(r1, x - kWave), acc);
+            for (; x < xhi; ++x) acc = fma(pr[x], rr[x], acc);
         }
     }
     __syncthreads();  // out may alias the staged rows

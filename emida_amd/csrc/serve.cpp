@@ -13,8 +13,8 @@
 // for a plan that has requests waiting.  Batches go round-robin over several streams and overlap on the device.
 // Each (slot, batch size) pipeline - kernels plus the copies of the results into the
 // slot's pinned buffers - is captured once in a hipGraph and replayed; batch sizes
-// are rounded up to a power of two for this, the extra slices computing on stale
-// slot contents whose results nobody reads.  One
+// are rounded up to a power of two or 3/2 of one for this (graph_batch), the extra
+// slices computing on stale slot contents whose results nobody reads.  One
 // completion thread per stream waits for its batches in order, hands results to the
 // requests and frees the slot and the staging buffers.
 #include <chrono>
@@ -26,6 +26,19 @@
 #include "offset.hpp"
 
 namespace {
+
+// Batch size a graph is captured for: the smallest of 1, 2, 3, 4, 6, 8, 12, 16, 24, ...
+// (powers of two and 3/2 of them) that holds n, so a replay computes at most a third
+// more slices than it needs (a power-of-two grid wastes up to all of them again) with
+// two graphs per octave of sizes.
+uint32_t graph_batch(uint32_t n) {
+    uint32_t p = 1;
+    while (p < n) {
+        if (p >= 2 && p / 2 * 3 >= n) return p / 2 * 3;
+        p *= 2;
+    }
+    return p;
+}
 
 uint64_t now_ns() {
     return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
@@ -59,7 +72,7 @@ struct emida_server_s {
         double* h_coefs = nullptr;
         hipEvent_t done = nullptr;
         size_t bytes = 0;
-        std::map<uint32_t, hipGraphExec_t> graphs;  // by (power-of-two) batch size
+        std::map<uint32_t, hipGraphExec_t> graphs;  // by captured batch size (graph_batch)
     };
     struct plan_rt {
         std::unique_ptr<emida_offset_s> off;
@@ -415,11 +428,7 @@ struct emida_server_s {
         emida_offset_s& o = *p.off;
         const uint32_t n = (uint32_t)reqs.size();
         uint32_t nr = n;  // patterns the launch computes
-        if (cfg.use_graphs) {
-            nr = 1;
-            while (nr < n) nr *= 2;
-            nr = std::min(nr, p.max_batch);
-        }
+        if (cfg.use_graphs) nr = std::min(graph_batch(n), p.max_batch);
         const size_t tn = (size_t)o.nb * nr;
         const uint64_t t = now_ns();
         for (uint32_t i = 0; i < n; ++i) {

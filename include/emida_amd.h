@@ -319,10 +319,10 @@ int emida_generate_hanning(emida_type T, size_t size, void* out);
  * their total size stays within hbm_budget; batches overlap on `streams` streams; a
  * (slot, batch size) pipeline is captured in a hipGraph once and replayed.  Each
  * request's results are the same as those of a lone request (the per-slice work does
- * not depend on the other slices of a batch).  Graphs exist per power-of-two batch
- * size; a batch of n requests runs the graph of the next power of two, the extra
- * slices computing on stale slot contents whose results are discarded.  Patterns are
- * uint16.
+ * not depend on the other slices of a batch).  Graphs exist per batch size on the grid
+ * 1, 2, 3, 4, 6, 8, 12, 16, 24, ... (powers of two and 3/2 of them); a batch of n
+ * requests runs the graph of the next size, the extra slices computing on stale slot
+ * contents whose results are discarded.  Patterns are uint16.
  * ====================================================================== */
 typedef struct emida_server_s emida_server;
 

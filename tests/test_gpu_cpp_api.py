@@ -3,14 +3,17 @@
     include/emida/gpu_offset.hpp and the reference's threaded loader / GPU / finalize
     pipeline;
   * test_cross_corr_fft: the reference's cross_corr_fft test (test/cross_corr_fft_test.cpp)
-    with assertions, get_cu_pic / get_cu_temp and the public streams."""
+    with assertions, get_cu_pic / get_cu_temp and the public streams;
+  * test_threads: the request server under four submitting threads and a statistics
+    reader (every served result equal to the same request served alone) and the file
+    pipeline with four loaders, results in completion order and one unreadable file."""
 import os
 import subprocess
 
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-BINS = [os.path.join(ROOT, "tests", "cpp", n) for n in ("test_gpu_offset", "test_cross_corr_fft")]
+BINS = [os.path.join(ROOT, "tests", "cpp", n) for n in ("test_gpu_offset", "test_cross_corr_fft", "test_threads")]
 
 
 @pytest.mark.gpu

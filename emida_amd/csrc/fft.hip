@@ -267,50 +267,51 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
         }
     }
     __syncthreads();  // row spectra consumed; slots become column scratch
+    // the first half of this column's reference spectrum, loaded here so that the
+    // forward transform covers its latency (all of it: 2% slower, more registers)
+    constexpr int PF = R / 2;
+    C spec[R];
+    if constexpr (PIPE) {
+        if (c < S) {
+            const C* rc = a.spectra + ((size_t)bn * P::NC + c) * M + lane * R;
+#pragma unroll
+            for (int r = 0; r < PF; ++r) spec[r] = rc[r];
+        }
+    }
     if (c < S) {
         C* slot = B + c * CS;
         fft_fwd<T, R, L, M, true>(x, lane, slot, tw);
         if (c == 0) {
             // split the packed real columns: P0[k] = (Z[k] + conj(Z[-k]))/2,
-            // PS[k] = (Z[k] - conj(Z[-k]))/(2i)
+            // PS[k] = (Z[k] - conj(Z[-k]))/(2i), then H0 + i HS (PIPE) or the two
+            // reference spectra, element by element
 #pragma unroll
             for (int i = 0; i < Q; ++i)
 #pragma unroll
                 for (int k2 = 0; k2 < L; ++k2) slot[(lane + L * i) + R * k2] = x[i * L + k2];
             wave_sync();
-            C p0[R], ps[R];
+            const size_t c0 = ((size_t)bn * P::NC + 0) * M + lane * R, cs = ((size_t)bn * P::NC + S) * M + lane * R;
 #pragma unroll
             for (int i = 0; i < Q; ++i)
 #pragma unroll
                 for (int k2 = 0; k2 < L; ++k2) {
-                    const int k = (lane + L * i) + R * k2;
-                    const C z = x[i * L + k2], zm = slot[(M - k) & (M - 1)];
-                    p0[i * L + k2] = C{z.re + zm.re, z.im - zm.im};
-                    ps[i * L + k2] = C{z.im + zm.im, zm.re - z.re};
+                    const int k = (lane + L * i) + R * k2, r = i * L + k2;
+                    const C z = x[r], zm = slot[(M - k) & (M - 1)];
+                    const C p0 = C{z.re + zm.re, z.im - zm.im}, ps = C{z.im + zm.im, zm.re - z.re};
+                    if constexpr (PIPE) {
+                        const C h0 = hadamard(p0, r < PF ? spec[r] : a.spectra[c0 + r]), hs = hadamard(ps, a.spectra[cs + r]);
+                        x[r] = C{h0.re - hs.im, h0.im + hs.re};  // H0 + i HS
+                    } else {
+                        a.spectra_out[c0 + r] = C{p0.re * sc, p0.im * sc};
+                        a.spectra_out[cs + r] = C{ps.re * sc, ps.im * sc};
+                    }
                 }
             wave_sync();
-            if constexpr (PIPE) {
-                const C* r0 = a.spectra + ((size_t)bn * P::NC + 0) * M + lane * R;
-                const C* rs = a.spectra + ((size_t)bn * P::NC + S) * M + lane * R;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const C h0 = hadamard(p0[r], r0[r]), hs = hadamard(ps[r], rs[r]);
-                    x[r] = C{h0.re - hs.im, h0.im + hs.re};  // H0 + i HS
-                }
-            } else {
-                C* o0 = a.spectra_out + ((size_t)bn * P::NC + 0) * M + lane * R;
-                C* os = a.spectra_out + ((size_t)bn * P::NC + S) * M + lane * R;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    o0[r] = C{p0[r].re * sc, p0[r].im * sc};
-                    os[r] = C{ps[r].re * sc, ps[r].im * sc};
-                }
-            }
         } else {
             if constexpr (PIPE) {
                 const C* rc = a.spectra + ((size_t)bn * P::NC + c) * M + lane * R;
 #pragma unroll
-                for (int r = 0; r < R; ++r) x[r] = hadamard(x[r], rc[r]);
+                for (int r = 0; r < R; ++r) x[r] = hadamard(x[r], r < PF ? spec[r] : rc[r]);
             } else {
                 C* oc = a.spectra_out + ((size_t)bn * P::NC + c) * M + lane * R;
 #pragma unroll

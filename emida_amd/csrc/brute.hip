@@ -65,20 +65,24 @@ __global__ __launch_bounds__(kBruteTX* kBruteTY) void cross_corr_kernel(const T*
     res[(size_t)sn * res_size.area() + (size_t)ry * res_size.x + rx] = sum;
 }
 
-// The same sums, NX consecutive x-shifts per thread, for slices staged in LDS: the
-// pattern rows are stored with NX - 1 zeros on each side, so an NX-value window slides
-// along the row (one new pattern value and one reference value per step feed NX fmas).
-// Each thread walks the union of its shifts' overlaps; a term outside a shift's own
-// overlap multiplies an exact zero and leaves its sum unchanged (finite values), and
-// every sum still sees its own terms y outer, x inner.  Indices: the window reads
-// pattern column x + sx0 + t, t < NX, for x in the union, which stays in
-// [-(NX - 1), S + NX - 1).
+// The same sums, NX consecutive x-shifts per thread, for slices staged in LDS: an
+// NX-value window slides along the pattern row (one new pattern value and one
+// reference value per column feed NX fmas).  Each thread walks the union of its
+// shifts' overlaps, rounded up to whole blocks of NX columns, so the window moves in
+// registers by compile-time index within a block; the pattern rows are stored with
+// NX - 1 zeros before and 2 NX - 2 after them and the reference rows with NX zeros
+// after them, so a term outside a shift's own overlap multiplies an exact zero and
+// leaves its sum unchanged (finite values), and every sum still sees its own terms
+// y outer, x inner.  Indices: the window reads pattern column x + sx0 + t, t < NX,
+// for x in [xu0, xu0 + NX * blocks) with xu0 + NX * blocks <= xu1 + NX - 1, which
+// stays in [-(NX - 1), S + 2 NX - 2); reference columns stay below S + NX - 1.
 constexpr int kXN = 8;  // x-shifts per thread (4: 0.146 ms, 16: 0.167 ms at 32 x 32 slices, 63 x 63 windows; 8: 0.130 ms)
 constexpr int kXNTX = 64 / kXN, kXNTY = 256 / kXNTX;
+constexpr int kXNPadL = kXN - 1, kXNPadR = 2 * kXN - 2;
 
 template <class T>
 __host__ __device__ inline size_t xn_lds_elems(size2 size) {
-    return (size_t)size.y * (size.x + 2 * (kXN - 1)) + (size_t)size.y * size.x;
+    return (size_t)size.y * (size.x + kXNPadL + kXNPadR) + (size_t)size.y * (size.x + kXN);
 }
 
 template <class T>
@@ -86,20 +90,23 @@ __global__ __launch_bounds__(kXNTX* kXNTY) void cross_corr_x8_kernel(const T* __
                                                                     const T* __restrict__ ref,
                                                                     T* __restrict__ res, size2 size,
                                                                     size2 res_size, uint32_t ref_slices) {
-    constexpr int NX = kXN, PAD = kXN - 1;
+    constexpr int NX = kXN;
     extern __shared__ unsigned char smem_raw[];
     const uint32_t sn = blockIdx.z;
-    const int S = (int)size.x, SY = (int)size.y, W = S + 2 * PAD;
+    const int S = (int)size.x, SY = (int)size.y, W = S + kXNPadL + kXNPadR, SQ = S + NX;
     T* sp = reinterpret_cast<T*>(smem_raw);  // SY rows of W: zeros, pattern row, zeros
-    T* sq = sp + (size_t)SY * W;             // SY rows of S
+    T* sq = sp + (size_t)SY * W;             // SY rows of SQ: reference row, zeros
     const T* p = pics + (size_t)sn * size.area();
     const T* q = ref + (size_t)(sn % ref_slices) * size.area();
     const int tid = threadIdx.y * kXNTX + threadIdx.x;
     for (int i = tid; i < SY * W; i += kXNTX * kXNTY) {
-        const int y = i / W, c = i % W - PAD;
+        const int y = i / W, c = i % W - kXNPadL;
         sp[i] = (c >= 0 && c < S) ? p[(size_t)y * S + c] : T(0);
     }
-    for (int i = tid; i < SY * S; i += kXNTX * kXNTY) sq[i] = q[i];
+    for (int i = tid; i < SY * SQ; i += kXNTX * kXNTY) {
+        const int y = i / SQ, c = i % SQ;
+        sq[i] = c < S ? q[(size_t)y * S + c] : T(0);
+    }
     __syncthreads();
     const uint32_t rx0 = (blockIdx.x * kXNTX + threadIdx.x) * NX;
     const uint32_t ry = blockIdx.y * kXNTY + threadIdx.y;
@@ -112,19 +119,26 @@ __global__ __launch_bounds__(kXNTX* kXNTY) void cross_corr_x8_kernel(const T* __
 #pragma unroll
     for (int t = 0; t < NX; ++t) acc[t] = T(0);
     if (xu0 < xu1) {
+        const int blocks = (xu1 - xu0 + NX - 1) / NX;
         for (int y = max(-sy, 0); y < min(SY, SY - sy); ++y) {
-            const T* pr = sp + (size_t)(y + sy) * W + PAD + sx0;  // pr[x + t]: column x + sx0 + t
-            const T* qr = sq + (size_t)y * S;
-            T w[NX];
+            const T* pr = sp + (size_t)(y + sy) * W + kXNPadL + sx0 + xu0;  // pr[j + t]: column xu0 + j + sx0 + t
+            const T* qr = sq + (size_t)y * SQ + xu0;
+            T w[2 * NX - 1];
 #pragma unroll
-            for (int t = 0; t < NX - 1; ++t) w[t] = pr[xu0 + t];
-            for (int x = xu0; x < xu1; ++x) {
-                w[NX - 1] = pr[x + NX - 1];
-                const T qv = qr[x];
+            for (int t = 0; t < NX - 1; ++t) w[t] = pr[t];
+            for (int b = 0; b < blocks; ++b, pr += NX, qr += NX) {
+                T qv[NX];
 #pragma unroll
-                for (int t = 0; t < NX; ++t) acc[t] = mac<T>(w[t], qv, acc[t]);
+                for (int u = 0; u < NX; ++u) {
+                    w[NX - 1 + u] = pr[NX - 1 + u];
+                    qv[u] = qr[u];
+                }
 #pragma unroll
-                for (int t = 0; t < NX - 1; ++t) w[t] = w[t + 1];
+                for (int u = 0; u < NX; ++u)
+#pragma unroll
+                    for (int t = 0; t < NX; ++t) acc[t] = mac<T>(w[u + t], qv[u], acc[t]);
+#pragma unroll
+                for (int t = 0; t < NX - 1; ++t) w[t] = w[NX + t];
             }
         }
     }

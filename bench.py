@@ -24,8 +24,8 @@ Also reported:
                 p50/p99 latency at 2000 requests/s offered, saturation patterns/s, and
                 the plain FP64 brute form served the same way;
   precision_form (default workload) the precision form's shape (100 ROIs of 128 x 128,
-                fitsize 7, batch 128): refined fp16-split matrix-core brute against the
-                plain FP64 brute form;
+                fitsize 7, batch 128): refined fp16 matrix-core brute (the default
+                mode) against the plain FP64 brute form;
   batch latency p50/p99 time for one batch to complete when submitted alone.
 """
 import argparse
@@ -311,7 +311,7 @@ def serve_plans(cfg, plain_fp64=False):
             if plain_fp64:
                 out.append((r, k, np.float64, emida_amd.CROSS_POLICY_BRUTE, 0))
             elif r == min(cfg["radii"]) or 2 * r > emida_amd.fft_max_slice(np.float32):
-                out.append((r, k, np.float32, emida_amd.CROSS_POLICY_BRUTE_MFMA, emida_amd.MFMA_F16_SPLIT))
+                out.append((r, k, np.float32, emida_amd.CROSS_POLICY_BRUTE_MFMA, emida_amd.MFMA_AUTO))
             else:
                 out.append((r, k, np.float32, emida_amd.CROSS_POLICY_FFT, 0))
     return out
@@ -386,11 +386,11 @@ def serve_main(args, cfg):
     out = {"metric": METRIC, "value": round(sat_rate, 2), "unit": "patterns/s", "n_gpus": 1,
            "steps": n_sat, "warmup": 40 * len(plans), "ms_per_step": round(1e3 / sat_rate, 4),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f32 FFT (r32, r64) + f16 split MFMA brute (r16), f32 accumulate", "data": "synthetic",
+           "dtype": "f32 FFT (r32, r64) + f16 MFMA brute screen with exact f32 peak (r16), f32 accumulate", "data": "synthetic",
            "config": {"workload": args.workload, "pattern": f"{cfg['size']}x{cfg['size']} uint16",
                       "radii": list(cfg["radii"]), "roi_grids": [k * k for k in cfg["grids"]], "fitsize": fit,
                       "plans": [f"r{p['r']}/{p['k'] * p['k']} ROIs/" +
-                                ("fft" if p["pol"] == emida_amd.CROSS_POLICY_FFT else "mfma-f16-split")
+                                ("fft" if p["pol"] == emida_amd.CROSS_POLICY_FFT else "mfma-f16-refined")
                                 for p in plans],
                       "offered_rate": cfg["rate"], "offered_seconds": args.duration,
                       "hbm_budget": cfg["hbm_budget"], "max_batch": args.serve_batch, "streams": 4,
@@ -489,7 +489,7 @@ def scan_main(args, cfg):
     fft_rate, fft_lat, fft_out = stream(emida_amd.CROSS_POLICY_FFT, 0)
     # matrix-core screen with the exact peak in double: the FP64 brute results
     mfma_rate, mfma_lat, mfma_out = stream(emida_amd.CROSS_POLICY_BRUTE_MFMA, emida_amd.MFMA_BF16_SPLIT, np.float64)
-    f16_rate, f16_lat, f16_out = stream(emida_amd.CROSS_POLICY_BRUTE_MFMA, emida_amd.MFMA_F16_SPLIT, np.float64)
+    f16_rate, f16_lat, f16_out = stream(emida_amd.CROSS_POLICY_BRUTE_MFMA, emida_amd.MFMA_AUTO, np.float64)
 
     # FP64 reference offsets of the subset, brute force on the GPU, and the FP64
     # correlation's peak statistics (the well-conditioned criterion)
@@ -517,7 +517,7 @@ def scan_main(args, cfg):
 
     out = {"metric": METRIC, "value": round(fft_rate, 2), "unit": "patterns/s", "n_gpus": 1, "steps": n,
            "warmup": min(64, n), "ms_per_step": round(1e3 / fft_rate, 4), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f32 FFT; bf16 / f16 split MFMA brute screen (f32 accumulate) + exact f64 peak",
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32 FFT; bf16 split / f16 MFMA brute screen (f32 accumulate) + exact f64 peak",
            "data": "synthetic",
            "config": {"workload": args.workload, "pattern": f"{cfg['size']}x{cfg['size']} uint16",
                       "patterns": n, "rois": len(begins), "roi": f"{2 * r}x{2 * r}", "fitsize": fit,
@@ -531,7 +531,7 @@ def scan_main(args, cfg):
                                "roi_offsets_per_s": round(mfma_rate * len(begins)),
                                "p50_pattern_latency_ms": round(float(np.percentile(mfma_lat, 50)), 3),
                                "err_vs_fp64": err(mfma_out)},
-           "mfma_f16_split": {"patterns_per_s": round(f16_rate, 1),
+           "mfma_f16": {"patterns_per_s": round(f16_rate, 1),
                               "roi_offsets_per_s": round(f16_rate * len(begins)),
                               "p50_pattern_latency_ms": round(float(np.percentile(f16_lat, 50)), 3),
                               "err_vs_fp64": err(f16_out)},
@@ -965,9 +965,10 @@ def served_block(args, cfg, ps, begins, slice_size, cross):
 
 def precision_block(args, ps):
     """The precision form at its own shape (873 x 873, 100 ROIs of 128 x 128, the full
-    255 x 255 window, fitsize 7, batch 128): fp16-split matrix-core screen with the exact
-    double peak, against the plain FP64 brute form on the same resident patterns; both
-    give the same results (checked here on one batch)."""
+    255 x 255 window, fitsize 7, batch 128): matrix-core screen in the default mode (fp16
+    operands for uint16 patterns) with the exact double peak, against the plain FP64
+    brute form on the same resident patterns; both give the same results (checked here
+    on one batch)."""
     import emida_amd
     from emida_amd import synth
 
@@ -979,7 +980,7 @@ def precision_block(args, ps):
     d_pics = emida_amd.DeviceBuffer.from_array(pics)
     total = len(begins) * batch
     res = {}
-    for name, policy, steps in (("refined_f16_split", emida_amd.CROSS_POLICY_BRUTE_MFMA, args.precision_steps),
+    for name, policy, steps in (("refined_f16", emida_amd.CROSS_POLICY_BRUTE_MFMA, args.precision_steps),
                                 ("plain_fp64_brute", emida_amd.CROSS_POLICY_BRUTE, max(1, args.precision_steps // 3))):
         with emida_amd.GpuOffset(np.float64, np.uint16, ps.size, begins, s2, cross, batch, policy, fit) as g:
             g.allocate_memory(ps.ref)
@@ -992,12 +993,12 @@ def precision_block(args, ps):
             emida_amd.synchronize()
             el = time.perf_counter() - t0
             res[name] = (steps * batch / el, d_offs.download(np.float64, 2 * total))
-    same = bool(np.array_equal(res["refined_f16_split"][1], res["plain_fp64_brute"][1], equal_nan=True))
+    same = bool(np.array_equal(res["refined_f16"][1], res["plain_fp64_brute"][1], equal_nan=True))
     return {"shape": f"{cfg3['size']}x{cfg3['size']}, {len(begins)} ROIs of {2 * r}x{2 * r}, window "
                      f"{cross[0]}x{cross[1]}, fitsize {fit}, batch {batch}",
-            "refined_f16_split_patterns_per_s": round(res["refined_f16_split"][0], 1),
+            "refined_f16_patterns_per_s": round(res["refined_f16"][0], 1),
             "plain_fp64_brute_patterns_per_s": round(res["plain_fp64_brute"][0], 2),
-            "refined_over_plain_fp64": round(res["refined_f16_split"][0] / res["plain_fp64_brute"][0], 2),
+            "refined_over_plain_fp64": round(res["refined_f16"][0] / res["plain_fp64_brute"][0], 2),
             "results_equal_fp64_brute": same}
 
 

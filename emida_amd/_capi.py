@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(_HERE, "libemida_amd.so")
 
 F32, F64, I32, U16 = 0, 1, 2, 3
 CROSS_POLICY_BRUTE, CROSS_POLICY_FFT, CROSS_POLICY_BRUTE_MFMA = 0, 1, 2
-MFMA_BF16, MFMA_BF16_SPLIT, MFMA_F16, MFMA_F16_SPLIT = 0, 1, 2, 3
+MFMA_AUTO, MFMA_BF16, MFMA_BF16_SPLIT, MFMA_F16, MFMA_F16_SPLIT = -1, 0, 1, 2, 3
 OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_UNSUPPORTED, ERR_NOMEM = range(6)
 
 _TYPE_OF = {np.dtype(np.float32): F32, np.dtype(np.float64): F64, np.dtype(np.int32): I32,
@@ -113,6 +113,7 @@ def lib():
             "emida_offset_get_cu_cross_res": ([vp, ctypes.POINTER(vp)], i32),
             "emida_offset_get_streams": ([vp, ctypes.POINTER(vp), ctypes.POINTER(vp)], i32),
             "emida_offset_set_mfma_mode": ([vp, i32], i32),
+            "emida_offset_get_mfma_mode": ([vp, ctypes.POINTER(i32)], i32),
             "emida_offset_set_mfma_refine": ([vp, i32], i32),
             "emida_offset_set_timing": ([vp, i32], i32),
             "emida_offset_get_timing": ([vp, ctypes.POINTER(ctypes.c_char_p), dp, ctypes.POINTER(u64)], i32),

@@ -44,9 +44,6 @@ int emida_offset_create(emida_offset** out, emida_type T, emida_type IN, emida_s
         h->sum_bytes = IN == EMIDA_U16 ? 4 : type_size(IN);
         if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA) {
             check_mfma_shape(S2(slice_size), S2(cross_size));
-            // fp16 operands hold |prepared value| < 131008 only, which uint16 patterns
-            // guarantee; float and double patterns default to bf16 (float's range)
-            if (IN != EMIDA_U16) h->mfma_mode = EMIDA_MFMA_BF16_SPLIT;
         }
         if (policy != EMIDA_CROSS_POLICY_FFT) {
             if (cross_size.x == 0 || cross_size.y == 0) throw arg_error("cross size must be positive");
@@ -195,12 +192,19 @@ int emida_offset_set_mfma_mode(emida_offset* h, emida_mfma_mode mode) {
     return guarded([&] {
         if (!h) throw arg_error("null object");
         if (h->allocated) throw state_error("set_mfma_mode must precede allocate_memory");
-        with_HT(mode, [](auto) {});
+        if (mode != EMIDA_MFMA_AUTO) with_HT(mode, [](auto) {});
         if ((mode == EMIDA_MFMA_F16 || mode == EMIDA_MFMA_F16_SPLIT) && h->IN != EMIDA_U16)
             throw unsupported_error(
                 "fp16 matrix-core operands hold prepared values below 131008 in magnitude, which only uint16 "
                 "patterns guarantee; use a bf16 mode for float or double patterns");
         h->mfma_mode = mode;
+    });
+}
+
+int emida_offset_get_mfma_mode(emida_offset* h, emida_mfma_mode* mode) {
+    return guarded([&] {
+        if (!h || !mode) throw arg_error("null argument");
+        *mode = h->mfma_mode_used();
     });
 }
 

@@ -153,7 +153,7 @@ struct emida_offset_s {
     void* d_ref = nullptr;     // brute: prepared reference slices; FFT: spectra
     void* d_slices = nullptr;  // brute: prepared pattern slices
     void* d_corr = nullptr;    // brute: correlation buffers
-    emida_mfma_mode mfma_mode = EMIDA_MFMA_F16_SPLIT;
+    emida_mfma_mode mfma_mode = EMIDA_MFMA_AUTO;  // as set; mfma_mode_used() is the one in effect
     bool mfma_refine = true;   // matrix-core policy: exact peak from T-precision values
     void* d_ref_lo = nullptr;  // matrix-core policy: lo halves of the reference slices
     void* d_ref_t = nullptr;   // matrix-core policy with refinement: T-precision reference slices
@@ -239,7 +239,7 @@ struct emida_offset_s {
             }
             if (IN != EMIDA_U16) {  // uint16 patterns are prepared by the matrix-core kernel itself
                 parts[3] = al256(tn * slice.area() * 2);
-                if (mfma_split(mfma_mode)) parts[4] = parts[3];
+                if (mfma_split(mfma_mode_used())) parts[4] = parts[3];
             }
             parts[5] = al256(tn * cross.y * sizeof(uint32_t));
         } else if (fftg) {
@@ -368,7 +368,7 @@ struct emida_offset_s {
                                                src, slice, slice, nb, nb, compute);
                 } else if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA) {
                     launch_sum<IT, ST>(pic, sums, d_begins, src, slice, nb, nb, compute);
-                    with_HT(mfma_mode, [&](auto ht) {
+                    with_HT(mfma_mode_used(), [&](auto ht) {
                         using HT = typename decltype(ht)::type;
                         launch_prepare_split<IT, ST, HT>(pic, (HT*)d_ref, (HT*)d_ref_lo, (const float*)float_hx(),
                                                          (const float*)float_hy(), sums, d_begins, src, slice, nb, nb,
@@ -404,6 +404,18 @@ struct emida_offset_s {
         });
         EMIDA_HIP(hipStreamSynchronize(compute));
         EMIDA_HIP(hipFree(d_temp));
+    }
+
+    // The matrix-core mode in effect: the one set, or for EMIDA_MFMA_AUTO the plain
+    // half type when the peak is refined (exact results in every mode; one product in
+    // three, and a third of the fp32 accumulations, which keeps the screen's bound
+    // smaller than a split mode's) and the split one when the screen values are the
+    // results; fp16 for uint16 patterns, bf16 (float's range) for float and double.
+    emida_mfma_mode mfma_mode_used() const {
+        if (mfma_mode != EMIDA_MFMA_AUTO) return mfma_mode;
+        const bool u16 = IN == EMIDA_U16;
+        if (mfma_refine) return u16 ? EMIDA_MFMA_F16 : EMIDA_MFMA_BF16;
+        return u16 ? EMIDA_MFMA_F16_SPLIT : EMIDA_MFMA_BF16_SPLIT;
     }
 
     // resident_only: the reference data alone (for callers that bring their own
@@ -452,7 +464,7 @@ struct emida_offset_s {
         } else if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA) {
             const size_t half = 2;  // bf16 / fp16
             d_ref = alloc((size_t)nb * slice.area() * half);
-            if (mfma_split(mfma_mode)) d_ref_lo = alloc((size_t)nb * slice.area() * half);
+            if (mfma_split(mfma_mode_used())) d_ref_lo = alloc((size_t)nb * slice.area() * half);
             if (mfma_refine) {
                 d_ref_t = alloc((size_t)nb * slice.area() * t_bytes());
                 d_rn2 = alloc((size_t)nb * sizeof(double));
@@ -468,7 +480,7 @@ struct emida_offset_s {
                 }
                 if (IN != EMIDA_U16) {
                     d_pic_hi = alloc(scratch_total * slice.area() * half);
-                    if (mfma_split(mfma_mode)) d_pic_lo = alloc(scratch_total * slice.area() * half);
+                    if (mfma_split(mfma_mode_used())) d_pic_lo = alloc(scratch_total * slice.area() * half);
                 }
                 d_smax = alloc(scratch_total * cross.y * sizeof(uint32_t));
             }
@@ -627,9 +639,9 @@ struct emida_offset_s {
                         lists.pn2 = (const double*)sc.pn2;
                         lists.rn2 = (const double*)d_rn2;
                         lists.z0 = (const double*)sc.pn2 + tn;
-                        mfma_refine_bounds(mfma_mode, slice, T == EMIDA_F64, lists.tol_rel, lists.tol_abs);
+                        mfma_refine_bounds(mfma_mode_used(), slice, T == EMIDA_F64, lists.tol_rel, lists.tol_abs);
                     }
-                    with_HT(mfma_mode, [&](auto ht) {
+                    with_HT(mfma_mode_used(), [&](auto ht) {
                         using HT = typename decltype(ht)::type;
                         double* pn2 = mfma_refine ? (double*)sc.pn2 : nullptr;
                         // uint16 patterns: the matrix-core kernel prepares the pattern rows as

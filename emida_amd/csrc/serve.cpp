@@ -549,7 +549,7 @@ int emida_server_add_plan(emida_server* s, emida_type T, emida_size2 src_size, c
         }
         std::unique_ptr<emida_offset_s> off(h);
         if (policy == EMIDA_CROSS_POLICY_BRUTE_MFMA) {
-            with_HT(mfma_mode, [](auto) {});
+            if (mfma_mode != EMIDA_MFMA_AUTO) with_HT(mfma_mode, [](auto) {});
             off->mfma_mode = mfma_mode;
         }
         *plan_id = s->add_plan(std::move(off), ref);

@@ -16,7 +16,8 @@ from . import _capi as C
 CROSS_POLICY_BRUTE = C.CROSS_POLICY_BRUTE
 CROSS_POLICY_FFT = C.CROSS_POLICY_FFT
 CROSS_POLICY_BRUTE_MFMA = C.CROSS_POLICY_BRUTE_MFMA
-MFMA_BF16, MFMA_BF16_SPLIT, MFMA_F16, MFMA_F16_SPLIT = C.MFMA_BF16, C.MFMA_BF16_SPLIT, C.MFMA_F16, C.MFMA_F16_SPLIT
+MFMA_AUTO, MFMA_BF16, MFMA_BF16_SPLIT, MFMA_F16, MFMA_F16_SPLIT = (C.MFMA_AUTO, C.MFMA_BF16, C.MFMA_BF16_SPLIT,
+                                                              C.MFMA_F16, C.MFMA_F16_SPLIT)
 
 
 def data_index_dtype(t):
@@ -99,9 +100,11 @@ class GpuOffset:
     slice_size, cross_size, src_size: (x, y); policy: CROSS_POLICY_BRUTE,
     CROSS_POLICY_FFT or CROSS_POLICY_BRUTE_MFMA (matrix-core brute correlation);
     s: fitsize 3, 5, 7 or 9; mfma_mode: MFMA_BF16, MFMA_BF16_SPLIT, MFMA_F16 or
-    MFMA_F16_SPLIT (None: MFMA_F16_SPLIT for uint16 pixels, MFMA_BF16_SPLIT otherwise;
-    the fp16 modes need uint16 pixels), and mfma_refine: exact peak equal to CROSS_POLICY_BRUTE's results
-    (False: float screen values, T must be float32); both for CROSS_POLICY_BRUTE_MFMA.
+    MFMA_F16_SPLIT (None or MFMA_AUTO: the plain mode when refining, the split one
+    otherwise, fp16 for uint16 pixels and bf16 for float pixels; the fp16 modes need
+    uint16 pixels; .mfma_mode is the mode in effect), and mfma_refine: exact peak equal
+    to CROSS_POLICY_BRUTE's results (False: float screen values, T must be float32);
+    both for CROSS_POLICY_BRUTE_MFMA.
     """
 
     def __init__(self, T, IN, src_size, begins, slice_size, cross_size, batch_size, policy=CROSS_POLICY_BRUTE,
@@ -121,12 +124,12 @@ class GpuOffset:
                                             C.size2(cross_size), self.batch_size, self.policy, self.s,
                                             int(in_buffers_size)))
         self._h = h
-        if mfma_mode is None:  # the library's default for this pixel type
-            mfma_mode = MFMA_F16_SPLIT if self.IN == np.uint16 else MFMA_BF16_SPLIT
         if self.policy == CROSS_POLICY_BRUTE_MFMA:
-            C.check(C.lib().emida_offset_set_mfma_mode(h, int(mfma_mode)))
+            C.check(C.lib().emida_offset_set_mfma_mode(h, int(MFMA_AUTO if mfma_mode is None else mfma_mode)))
             C.check(C.lib().emida_offset_set_mfma_refine(h, 1 if mfma_refine else 0))
-        self.mfma_mode = int(mfma_mode)
+        used = ctypes.c_int32()
+        C.check(C.lib().emida_offset_get_mfma_mode(h, ctypes.byref(used)))
+        self.mfma_mode = used.value
         info = C.OffsetInfo()
         C.check(C.lib().emida_offset_get_info(h, ctypes.byref(info)))
         self.total_slices = info.total_slices

@@ -12,7 +12,7 @@ import threading
 import numpy as np
 
 from . import _capi as C
-from .gpu_offset import CROSS_POLICY_BRUTE, MFMA_F16_SPLIT
+from .gpu_offset import CROSS_POLICY_BRUTE, MFMA_AUTO
 
 
 class Server:
@@ -51,7 +51,7 @@ class Server:
         self.close()
 
     def add_plan(self, T, ref, begins, slice_size, cross_size=None, policy=CROSS_POLICY_BRUTE, s=3,
-                 mfma_mode=MFMA_F16_SPLIT) -> int:
+                 mfma_mode=MFMA_AUTO) -> int:
         """ref: uint16 (rows, cols) reference pattern; begins: (x, y) ROI corners.
         Returns the plan id."""
         ref = np.ascontiguousarray(ref, dtype=np.uint16)

@@ -152,9 +152,16 @@ public:
     esize_t total_slices() const { return total_; }
     esize_t neigh_size() const { return neigh_; }
 
-    // Half type of CROSS_POLICY_BRUTE_MFMA; call before allocate_memory (no reference counterpart).
+    // Half type of CROSS_POLICY_BRUTE_MFMA (default EMIDA_MFMA_AUTO); call before
+    // allocate_memory (no reference counterpart).
     void set_mfma_mode(emida_mfma_mode mode) {
         detail::check(emida_offset_set_mfma_mode(h_, mode), "set_mfma_mode");
+    }
+    // The half type in effect (what EMIDA_MFMA_AUTO resolves to).
+    emida_mfma_mode mfma_mode() const {
+        emida_mfma_mode m = EMIDA_MFMA_AUTO;
+        detail::check(emida_offset_get_mfma_mode(h_, &m), "get_mfma_mode");
+        return m;
     }
     // Exact peak equal to CROSS_POLICY_BRUTE's results (default); before allocate_memory.
     void set_mfma_refine(bool enable) {

@@ -55,9 +55,14 @@ typedef enum {
  * representation error 2^-16 for bf16, 2^-22 for fp16).  fp16 operands are the
  * prepared values halved, so any |value| < 131008 fits: every uint16 input does, so
  * the fp16 modes are accepted for IN = U16 only (emida_offset_set_mfma_mode returns
- * EMIDA_ERR_UNSUPPORTED otherwise).  Default: F16_SPLIT for U16 patterns, BF16_SPLIT
- * for F32 and F64 patterns. */
+ * EMIDA_ERR_UNSUPPORTED otherwise).  EMIDA_MFMA_AUTO (the default) takes the plain
+ * mode when the peak is refined (the results are exact in every mode; one product
+ * instead of three, and a third of the fp32 accumulations keeps the screen's bound,
+ * and so the refinement's candidates, smaller than a split mode's) and the split mode
+ * when the screen values are the results: F16 / F16_SPLIT for U16 patterns, BF16 /
+ * BF16_SPLIT for F32 and F64 patterns. */
 typedef enum {
+    EMIDA_MFMA_AUTO = -1,
     EMIDA_MFMA_BF16 = 0,
     EMIDA_MFMA_BF16_SPLIT = 1,
     EMIDA_MFMA_F16 = 2,
@@ -182,10 +187,15 @@ int emida_offset_get_cu_cross_res(emida_offset* h, void** dptr);
  * allocate_memory. */
 int emida_offset_get_streams(emida_offset* h, void** in_stream, void** out_stream);
 
-/* Operand precision of the BRUTE_MFMA policy (default EMIDA_MFMA_F16_SPLIT for U16
- * patterns, EMIDA_MFMA_BF16_SPLIT for F32/F64 patterns; the fp16 modes need IN = U16);
- * call before emida_offset_allocate_memory.  No reference counterpart. */
+/* Operand precision of the BRUTE_MFMA policy (default EMIDA_MFMA_AUTO, see
+ * emida_mfma_mode; the fp16 modes need IN = U16); call before
+ * emida_offset_allocate_memory.  No reference counterpart. */
 int emida_offset_set_mfma_mode(emida_offset* h, emida_mfma_mode mode);
+
+/* The operand precision the BRUTE_MFMA policy runs with: the mode set, or what
+ * EMIDA_MFMA_AUTO resolves to for the object's pixel type and refinement setting.
+ * No reference counterpart. */
+int emida_offset_get_mfma_mode(emida_offset* h, emida_mfma_mode* mode);
 
 /* Exact peak for the BRUTE_MFMA policy (default on): the matrix-core correlation only
  * screens; every shift within its rounding bound of the screen maximum and the F x F
@@ -354,7 +364,7 @@ int emida_server_create(emida_server** out, const emida_server_config* cfg);
 void emida_server_destroy(emida_server* s);
 /* A plan: the arguments of emida_offset_create (IN = U16) plus the reference pattern;
  * its reference slices or spectra are prepared now and stay resident.  mfma_mode is
- * used by EMIDA_CROSS_POLICY_BRUTE_MFMA only. */
+ * used by EMIDA_CROSS_POLICY_BRUTE_MFMA only (EMIDA_MFMA_AUTO: fp16, refined). */
 int emida_server_add_plan(emida_server* s, emida_type T, emida_size2 src_size, const uint16_t* ref,
                           const emida_size2* begins, uint32_t n_begins, emida_size2 slice_size,
                           emida_size2 cross_size, emida_cross_policy policy, emida_mfma_mode mfma_mode, int fitsize,

@@ -51,10 +51,10 @@ def _oracle_fft_check(pattern, ref, size, begins, side, fit, offs, maxes_index):
 
 # ------------------------------------------------------------------ serving
 SERVE_SIZE = 873
-SERVE_PLANS = [  # (radius, grid k, T, policy, matrix-core mode)
-    (r, k, np.float32, MFMA if r == 16 else FFT, emida_amd.MFMA_F16_SPLIT)
+SERVE_PLANS = [  # (radius, grid k, T, policy, matrix-core mode): the serving workload's plans
+    (r, k, np.float32, MFMA if r == 16 else FFT, emida_amd.MFMA_AUTO)
     for r in (16, 32, 64) for k in (10, 15, 20)
-] + [(16, 10, np.float64, MFMA, emida_amd.MFMA_F16_SPLIT)]
+] + [(16, 10, np.float64, MFMA, emida_amd.MFMA_F16_SPLIT)]  # and a split-mode double plan
 
 
 @pytest.fixture(scope="module")

@@ -98,17 +98,30 @@ def test_create_matrix_core_policy_without_gpu():
 def test_fp16_matrix_core_modes_need_uint16_patterns():
     # a float or double pattern can hold prepared values beyond fp16's range (131008
     # after the halving), which would put inf into the screen: those patterns default
-    # to bf16 split and an explicit fp16 mode is refused before any device work
+    # to a bf16 mode and an explicit fp16 mode is refused before any device work
     for IN in (np.float32, np.float64):
         g = _create(T=np.float64, IN=IN, policy=emida_amd.CROSS_POLICY_BRUTE_MFMA)
-        assert g.mfma_mode == emida_amd.MFMA_BF16_SPLIT
+        assert g.mfma_mode == emida_amd.MFMA_BF16
         for mode in (emida_amd.MFMA_BF16, emida_amd.MFMA_BF16_SPLIT):
             assert _create(T=np.float64, IN=IN, policy=emida_amd.CROSS_POLICY_BRUTE_MFMA, mfma_mode=mode).mfma_mode == mode
         for mode in (emida_amd.MFMA_F16, emida_amd.MFMA_F16_SPLIT):
             with pytest.raises(emida_amd.EmidaError) as e:
                 _create(T=np.float64, IN=IN, policy=emida_amd.CROSS_POLICY_BRUTE_MFMA, mfma_mode=mode)
             assert e.value.code == C.ERR_UNSUPPORTED and "bf16" in str(e.value)
-    assert _create(T=np.float64, policy=emida_amd.CROSS_POLICY_BRUTE_MFMA).mfma_mode == emida_amd.MFMA_F16_SPLIT
+    assert _create(T=np.float64, policy=emida_amd.CROSS_POLICY_BRUTE_MFMA).mfma_mode == emida_amd.MFMA_F16
+
+
+def test_matrix_core_default_mode_follows_refinement():
+    # the default (MFMA_AUTO): the plain half type when the peak is refined (exact
+    # results; a third of the products and of the fp32 accumulations), the split one
+    # when the screen values are the results
+    mfma = emida_amd.CROSS_POLICY_BRUTE_MFMA
+    for IN, plain, split in ((np.uint16, emida_amd.MFMA_F16, emida_amd.MFMA_F16_SPLIT),
+                             (np.float32, emida_amd.MFMA_BF16, emida_amd.MFMA_BF16_SPLIT)):
+        assert _create(T=np.float32, IN=IN, policy=mfma).mfma_mode == plain
+        assert _create(T=np.float32, IN=IN, policy=mfma, mfma_refine=False).mfma_mode == split
+        assert _create(T=np.float32, IN=IN, policy=mfma, mfma_mode=emida_amd.MFMA_AUTO).mfma_mode == plain
+        assert _create(T=np.float32, IN=IN, policy=mfma, mfma_mode=split).mfma_mode == split
 
 
 def test_synthetic_grid_matches_benchmark_definition():

@@ -69,7 +69,7 @@ struct fft_plan {
     static constexpr int CS = ((need + 7) / 16) * 16 + 8;
     static constexpr int P_OFF = NC * CS - S * S / 2;  // prepared ROI (S*S reals) at the end
     static_assert(P_OFF >= (S / 2) * CS, "prepared ROI overlaps the row stage");
-    static constexpr size_t lds_bytes = (size_t)(NC * CS + M) * sizeof(cplx<T>);
+    static constexpr size_t lds_bytes = (size_t)(NC * CS + M + L) * sizeof(cplx<T>);  // + twiddles [L][R + 1]
     // waves per SIMD the LDS allows (160 KiB per CU, 4 SIMDs); registers are
     // budgeted for exactly that many
     static constexpr int WG_PER_CU = (int)((160 * 1024) / (lds_bytes + kStaticLds));
@@ -189,7 +189,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
         }
         double lv[kLsqPer];
         load_lsq<NT>(lv, a.lsq, kk, tid);
-        load_twiddles<T, M>(tw, a.twt, tid, NT);
+        load_twiddles_2d<T, M, R>(tw, a.twt, tid, NT);
         store_lsq<NT>(lsq_s, lv, kk, tid);
         T mean;
         if constexpr (std::is_same<IN, uint16_t>::value) {
@@ -217,7 +217,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
         for (int e = tid; e < s * s; e += NT) pre[e] = src[e];
         double lv[kLsqPer];
         load_lsq<NT>(lv, a.lsq, kk, tid);
-        load_twiddles<T, M>(tw, a.twt, tid, NT);
+        load_twiddles_2d<T, M, R>(tw, a.twt, tid, NT);
         store_lsq<NT>(lsq_s, lv, kk, tid);
     }
     __syncthreads();
@@ -237,7 +237,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
             }
         }
         C* slot = B + p * CS;
-        fft_fwd<T, R, L, M, true>(x, lane, slot, tw);
+        fft_fwd<T, R, L, M, true, true>(x, lane, slot, tw);
 #pragma unroll
         for (int i = 0; i < Q; ++i)
 #pragma unroll
@@ -280,7 +280,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
     }
     if (c < S) {
         C* slot = B + c * CS;
-        fft_fwd<T, R, L, M, true>(x, lane, slot, tw);
+        fft_fwd<T, R, L, M, true, true>(x, lane, slot, tw);
         if (c == 0) {
             // split the packed real columns: P0[k] = (Z[k] + conj(Z[-k]))/2,
             // PS[k] = (Z[k] - conj(Z[-k]))/(2i), then H0 + i HS (PIPE) or the two
@@ -319,7 +319,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
             }
         }
         if constexpr (PIPE) {
-            fft_inv<T, R, L, M>(x, lane, slot, tw);
+            fft_inv<T, R, L, M, true>(x, lane, slot, tw);
             if (c == 0) {
                 C* slot_s = B + S * CS;
 #pragma unroll
@@ -373,7 +373,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
         T vmax = -(T)__builtin_huge_val();
         if (q < S && (in_a || in_b)) {
             C* slot = B + q * CS;
-            fft_inv<T, R, L, M>(x, lane, slot, tw);
+            fft_inv<T, R, L, M, true>(x, lane, slot, tw);
             T* cout = a.corr_out ? a.corr_out + (size_t)sn * s2 * s2 : nullptr;
 #pragma unroll
             for (int n1 = 0; n1 < R; ++n1) {

@@ -51,7 +51,8 @@ static_assert(kNC * kBS <= kLow, "band exceeds its region");
 static_assert(kCol + kG * kColStride <= kSlots, "job-3 columns exceed the slots");
 static_assert(6 * 81 <= kNT, "one least-squares element per thread");
 static_assert(4 * 81 <= kNT, "four threads per neighbourhood value");
-constexpr size_t kLds = (size_t)(kSlots + kM) * sizeof(cplx<float>);
+constexpr int kTw2 = kL * (kR + 1);  // the transforms' twiddles [n][k], rows of kR + 1
+constexpr size_t kLds = (size_t)(kSlots + kM + kTw2) * sizeof(cplx<float>);
 
 // Packed inverse of output rows (2p, 2p + 1): frequency k of the row pair is
 // Ga[k] + i Gb[k] with G[k] = col[k] for k <= S and conj(col[M - k]) above (DC and
@@ -128,7 +129,8 @@ __global__ __launch_bounds__(kNT, 1) void fft256_kernel(fft_args<float, IN, SUM>
     using C = cplx<float>;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     C* B = reinterpret_cast<C*>(lds_raw);  // 64 slots of kCS
-    C* tw = B + kSlots;
+    C* tw = B + kSlots;    // W_M^m, m < M (the neighbourhood's direct sums)
+    C* tw2 = tw + kM;      // the same values [n][k] for the transforms
     __shared__ uint32_t red_u[kNT / kWave];
     __shared__ float sv[kNT / kWave];
     __shared__ uint32_t si[kNT / kWave];
@@ -183,6 +185,7 @@ __global__ __launch_bounds__(kNT, 1) void fft256_kernel(fft_args<float, IN, SUM>
     const int kk = PIPE ? a.fit * a.fit : 0;
     const double lsq_v = tid < 6 * kk ? a.lsq[tid] : 0.0;
     load_twiddles<float, kM>(tw, a.twt, tid, kNT);  // while the loads are in flight
+    load_twiddles_2d<float, kM, kR>(tw2, a.twt, tid, kNT);
     // mean of the ROI (FROM_PATTERN): every pixel is one (pair, row, column) of one lane
     float mean = 0.f;
     if constexpr (FROM_PATTERN) {
@@ -241,7 +244,7 @@ __global__ __launch_bounds__(kNT, 1) void fft256_kernel(fft_args<float, IN, SUM>
                 }
             }
             C* slot = B + p * kCS;
-            fft_fwd<float, kR, kL, kM, true>(x, lane, slot, tw);
+            fft_fwd<float, kR, kL, kM, true, true>(x, lane, slot, tw2);
 #pragma unroll
             for (int k2 = 0; k2 < kL; ++k2) slot[lane + kR * k2] = x[k2];
         }
@@ -288,7 +291,7 @@ __global__ __launch_bounds__(kNT, 1) void fft256_kernel(fft_args<float, IN, SUM>
         // the twiddles and their addresses are reloaded per transform rather than kept
         // live across the column jobs (registers hold the four columns' results)
         asm volatile("" ::: "memory");
-        fft_fwd<float, kR, kL, kM, true>(xs[j], lane, slot, tw);
+        fft_fwd<float, kR, kL, kM, true, true>(xs[j], lane, slot, tw2);
         if (c == 0) {
             // split the packed real columns: P0[k] = (Z[k] + conj(Z[-k]))/2,
             // PS[k] = (Z[k] - conj(Z[-k]))/(2i) (factors 1/2 omitted, as in fft.hip),
@@ -326,7 +329,7 @@ __global__ __launch_bounds__(kNT, 1) void fft256_kernel(fft_args<float, IN, SUM>
         }
         if constexpr (PIPE) {
             asm volatile("" ::: "memory");
-            fft_inv<float, kR, kL, kM>(xs[j], lane, slot, tw);
+            fft_inv<float, kR, kL, kM, true>(xs[j], lane, slot, tw2);
             if (j == kJ - 1) {
                 // job 3: column c = 96 + g, rows kL n1 + lane, to the LDS column store
                 C* col = B + kCol + g * kColStride;
@@ -375,7 +378,7 @@ __global__ __launch_bounds__(kNT, 1) void fft256_kernel(fft_args<float, IN, SUM>
             C x[kR];
             row_pair_load(x, lane, band + 2 * g, kBS);
             __syncthreads();  // the band is read: its space becomes transpose scratch
-            fft_inv<float, kR, kL, kM>(x, lane, scr, tw);
+            fft_inv<float, kR, kL, kM, true>(x, lane, scr, tw2);
             const int ya = y0 + 2 * g, yb = ya + 1;
             const bool in_a = ya <= s || ya > s + pad, in_b = yb <= s || yb > s + pad;
             const uint32_t ra = (uint32_t)(ya <= s ? ya : ya - pad) * s2;

@@ -95,15 +95,24 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Twiddle W_M^(n k) of the two-level transforms (n < L, k < R): from a table of
+// W_M^m (m < M), or, TW2D, from a table laid out [n][k] with rows of R + 1
+// (load_twiddles_2d), where the address is a per-lane base plus a compile-time offset
+// and the odd row stride keeps the lanes of a group on distinct banks.
+template <int R, bool TW2D>
+__device__ __forceinline__ int tw_index(int n, int k) {
+    return TW2D ? n * (R + 1) + k : n * k;
+}
+
 // Forward transform: lane n2 holds x[n1] = input[L*n1 + n2]; on return lane j holds
 // X[(j + L*i) + R*k2] in x[i*L + k2].  scr: L*(R+1) elements of this group.
 // HALF_ZERO: the inputs x[R/2..R-1] are zero padding.
-template <class T, int R, int L, int M, bool HALF_ZERO>
+template <class T, int R, int L, int M, bool HALF_ZERO, bool TW2D = false>
 __device__ __forceinline__ void fft_fwd(cplx<T> (&x)[R], int lane, cplx<T>* scr, const cplx<T>* tw) {
     constexpr int Q = R / L;
     dft_reg<T, R, false, HALF_ZERO>(x);
 #pragma unroll
-    for (int k1 = 1; k1 < R; ++k1) x[k1] = cmul(x[k1], tw[lane * k1]);
+    for (int k1 = 1; k1 < R; ++k1) x[k1] = cmul(x[k1], tw[tw_index<R, TW2D>(lane, k1)]);
 #pragma unroll
     for (int k1 = 0; k1 < R; ++k1) scr[lane * (R + 1) + k1] = x[k1];
     wave_sync();
@@ -123,7 +132,7 @@ __device__ __forceinline__ void fft_fwd(cplx<T> (&x)[R], int lane, cplx<T>* scr,
 
 // Inverse (unnormalised) transform from the frequency layout to natural order:
 // on return lane n2 holds output[L*n1 + n2] in x[n1].
-template <class T, int R, int L, int M>
+template <class T, int R, int L, int M, bool TW2D = false>
 __device__ __forceinline__ void fft_inv(cplx<T> (&x)[R], int lane, cplx<T>* scr, const cplx<T>* tw) {
     constexpr int Q = R / L;
 #pragma unroll
@@ -134,7 +143,8 @@ __device__ __forceinline__ void fft_inv(cplx<T> (&x)[R], int lane, cplx<T>* scr,
         dft_reg<T, L, true>(y);
         const int k1 = lane + L * i;
 #pragma unroll
-        for (int n2 = 0; n2 < L; ++n2) scr[n2 * (R + 1) + k1] = n2 == 0 ? y[0] : cmul_conj(y[n2], tw[n2 * k1]);
+        for (int n2 = 0; n2 < L; ++n2)
+            scr[n2 * (R + 1) + k1] = n2 == 0 ? y[0] : cmul_conj(y[n2], tw[tw_index<R, TW2D>(n2, k1)]);
     }
     wave_sync();
 #pragma unroll
@@ -197,6 +207,15 @@ struct fft_args {
 template <class T, int M>
 __device__ __forceinline__ void load_twiddles(cplx<T>* tw, const cplx<T>* table, int tid, int nt) {
     for (int t = tid; t < M; t += nt) tw[t] = table[M - 4 + t];
+}
+// The same values laid out [n][k] = W_M^(n k), n < M / R, k < R, rows of R + 1
+// (M + M / R elements).
+template <class T, int M, int R>
+__device__ __forceinline__ void load_twiddles_2d(cplx<T>* tw, const cplx<T>* table, int tid, int nt) {
+    for (int t = tid; t < M + M / R; t += nt) {
+        const int n = t / (R + 1), k = t % (R + 1);
+        tw[t] = table[M - 4 + (k < R ? n * k % M : 0)];
+    }
 }
 
 }  // namespace emida_amd

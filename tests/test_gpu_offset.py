@@ -192,7 +192,9 @@ def test_fft_float_873_config_subset_vs_fp64_oracle():
                                      np.float64, return_intermediates=True, return_stats=True)
     ok = O.well_conditioned(ec, 5, st)
     np.testing.assert_array_equal(maxes["index"][ok], em["index"][ok])
-    assert np.abs(offs - eo)[ok].max() < 2e-3
+    worst = np.abs(offs - eo)[ok].max()
+    print(f"873 x 873 subset, float FFT: worst well-conditioned ROI {worst:.2e} px, all ROIs {np.abs(offs - eo).max():.2e} px")
+    assert worst < 2e-3, worst
     assert np.abs(offs - eo).max() < 1e-2
 
 

@@ -141,6 +141,31 @@ __device__ __forceinline__ void store_lsq(double* dst, const double (&v)[N], int
         if (tid + q * NT < 6 * kk) dst[tid + q * NT] = v[q];
 }
 
+// Workgroup maximum / minimum for arrays used once per kernel: one barrier (the
+// trailing one that would guard the array's reuse is not needed).
+template <class T, int NT>
+__device__ __forceinline__ T block_max_once(T v, T* sv) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    if (threadIdx.x % kWave == 0) sv[threadIdx.x / kWave] = v;
+    __syncthreads();
+    v = sv[0];
+#pragma unroll
+    for (int k = 1; k < NT / kWave; ++k) v = fmax(v, sv[k]);
+    return v;
+}
+template <int NT>
+__device__ __forceinline__ uint32_t block_min_once_u32(uint32_t v, uint32_t* sv) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off));
+    if (threadIdx.x % kWave == 0) sv[threadIdx.x / kWave] = v;
+    __syncthreads();
+    v = sv[0];
+#pragma unroll
+    for (int k = 1; k < NT / kWave; ++k) v = min(v, sv[k]);
+    return v;
+}
+
 // PAD: the slice side is below M/2 (a runtime value); otherwise it is M/2 and every
 // index expression below folds to constants.
 template <class T, class IN, class SUM, int M, bool FROM_PATTERN, bool PIPE, bool PAD>
@@ -390,7 +415,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
                 }
             }
         }
-        const T best = block_max<T, NT>(vmax, sv);
+        const T best = block_max_once<T, NT>(vmax, sv);
         uint32_t bi = 0xffffffffu;
         if (q < S && (in_a || in_b) && vmax == best) {
 #pragma unroll
@@ -403,13 +428,16 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
                 }
             }
         }
-        bi = block_min_u32<NT>(bi, si);
+        bi = block_min_once_u32<NT>(bi, si);
+        // the rest of the slice is wave 0's (its lanes exchange through LDS with
+        // wave-level ordering only)
+        if (tid >= kWave) return;
 
         // ---- D. gather (FFT position policy, clamp to [0, 2s-1]) and fit
         const int fit = a.fit, r = (fit - 1) / 2;
         const size2 cross{s2 - 1, s2 - 1};
         const size2 mp = shift_pos(POLICY_FFT, bi, cross);
-        for (int t = tid; t < kk; t += NT) {
+        for (int t = tid; t < kk; t += kWave) {
             const int i = t % fit, j = t / fit;
             const size2 cc = clamp_pos((int)mp.x - r + i, (int)mp.y - r + j, cross);
             int qx = (int)((cc.x + s + 1) % s2), qy = (int)((cc.y + s + 1) % s2);
@@ -420,7 +448,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
             nbv[t] = v;
             if (a.neigh) a.neigh[(size_t)sn * kk + t] = v;
         }
-        __syncthreads();
+        wave_sync();
         if (tid < 6) {
 #pragma clang fp contract(off)
             const double* mrow = lsq_s + tid * kk;
@@ -428,7 +456,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
             for (int i = 0; i < kk; ++i) sum += mrow[i] * (double)nbv[i];
             cf[tid] = sum;
         }
-        __syncthreads();
+        wave_sync();
         if (tid == 0) {
 #pragma clang fp contract(off)
             if (a.maxes) {

@@ -141,31 +141,6 @@ __device__ __forceinline__ void store_lsq(double* dst, const double (&v)[N], int
         if (tid + q * NT < 6 * kk) dst[tid + q * NT] = v[q];
 }
 
-// Workgroup maximum / minimum for arrays used once per kernel: one barrier (the
-// trailing one that would guard the array's reuse is not needed).
-template <class T, int NT>
-__device__ __forceinline__ T block_max_once(T v, T* sv) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
-    if (threadIdx.x % kWave == 0) sv[threadIdx.x / kWave] = v;
-    __syncthreads();
-    v = sv[0];
-#pragma unroll
-    for (int k = 1; k < NT / kWave; ++k) v = fmax(v, sv[k]);
-    return v;
-}
-template <int NT>
-__device__ __forceinline__ uint32_t block_min_once_u32(uint32_t v, uint32_t* sv) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off));
-    if (threadIdx.x % kWave == 0) sv[threadIdx.x / kWave] = v;
-    __syncthreads();
-    v = sv[0];
-#pragma unroll
-    for (int k = 1; k < NT / kWave; ++k) v = min(v, sv[k]);
-    return v;
-}
-
 // PAD: the slice side is below M/2 (a runtime value); otherwise it is M/2 and every
 // index expression below folds to constants.
 template <class T, class IN, class SUM, int M, bool FROM_PATTERN, bool PIPE, bool PAD>

@@ -732,7 +732,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
               "(patterns would shard across GPUs with no exchange between them)", file=sys.stderr)
         return 2
     if args.copy_calibration_mb:

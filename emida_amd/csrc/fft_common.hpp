@@ -58,11 +58,69 @@ __host__ __device__ constexpr int brev_c(int v, int bits) {
     return r;
 }
 
-// In-register DFT of N <= 16 points, natural order in and out (radix-2
-// decimation in frequency; the final bit reversal only renames registers).
+// v * W_16^j for any j (W_16^(j + 8) = -W_16^j).
+template <class T, bool INV>
+__device__ __forceinline__ cplx<T> twiddle16_any(cplx<T> v, int j) {
+    j &= 15;
+    const cplx<T> w = twiddle16<T, INV>(v, j & 7);
+    return j >= 8 ? cplx<T>{-w.re, -w.im} : w;
+}
+
+// 4-point DFT of (a0, a1, a2, a3) in place (forward: W_4 = -i; INV: +i); HALF_ZERO:
+// a2 = a3 = 0 (not read).
+template <class T, bool INV, bool HALF_ZERO>
+__device__ __forceinline__ void dft4(cplx<T>& a0, cplx<T>& a1, cplx<T>& a2, cplx<T>& a3) {
+    if constexpr (HALF_ZERO) {
+        const cplx<T> t0 = a0, t1 = a1;
+        a0 = cadd(t0, t1);
+        a2 = csub(t0, t1);
+        // -i t1 forward, +i t1 inverse
+        a1 = INV ? cplx<T>{t0.re - t1.im, t0.im + t1.re} : cplx<T>{t0.re + t1.im, t0.im - t1.re};
+        a3 = INV ? cplx<T>{t0.re + t1.im, t0.im - t1.re} : cplx<T>{t0.re - t1.im, t0.im + t1.re};
+    } else {
+        const cplx<T> t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = csub(a1, a3);
+        a0 = cadd(t0, t2);
+        a2 = csub(t0, t2);
+        a1 = INV ? cplx<T>{t1.re - t3.im, t1.im + t3.re} : cplx<T>{t1.re + t3.im, t1.im - t3.re};
+        a3 = INV ? cplx<T>{t1.re + t3.im, t1.im - t3.re} : cplx<T>{t1.re - t3.im, t1.im + t3.re};
+    }
+}
+
+// In-register DFT of N <= 16 points, natural order in and out.  N = 16: radix 4
+// (n = n0 + 4 n1, k = k1 + 4 k0: 4-point DFTs over n1, twiddles W_16^(n0 k1), 4-point
+// DFTs over n0; 8 non-trivial twiddle products instead of radix 2's 10); smaller N:
+// radix-2 decimation in frequency.  The final digit reversal only renames registers.
 template <class T, int N, bool INV, bool HALF_ZERO = false>
 __device__ __forceinline__ void dft_reg(cplx<T> (&x)[N]) {
     static_assert(N >= 1 && N <= 16 && (N & (N - 1)) == 0, "N must be a power of two <= 16");
+    if constexpr (N == 16) {
+#pragma unroll
+        for (int n0 = 0; n0 < 4; ++n0) {
+            cplx<T> a0 = x[n0], a1 = x[n0 + 4], a2, a3;
+            if constexpr (!HALF_ZERO) {
+                a2 = x[n0 + 8];
+                a3 = x[n0 + 12];
+            }
+            dft4<T, INV, HALF_ZERO>(a0, a1, a2, a3);
+            x[n0] = a0;
+            x[n0 + 4] = twiddle16_any<T, INV>(a1, n0);
+            x[n0 + 8] = twiddle16_any<T, INV>(a2, 2 * n0);
+            x[n0 + 12] = twiddle16_any<T, INV>(a3, 3 * n0);
+        }
+        cplx<T> y[16];
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            cplx<T> b0 = x[4 * k1], b1 = x[4 * k1 + 1], b2 = x[4 * k1 + 2], b3 = x[4 * k1 + 3];
+            dft4<T, INV, false>(b0, b1, b2, b3);
+            y[k1] = b0;
+            y[k1 + 4] = b1;
+            y[k1 + 8] = b2;
+            y[k1 + 12] = b3;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = y[i];
+        return;
+    }
     if constexpr (HALF_ZERO && N >= 2) {
         // first pass with x[N/2..N-1] = 0: x[k] unchanged, x[k + N/2] = x[k] W^k
 #pragma unroll

@@ -249,20 +249,30 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
     C x[R];
     const int c = grp;  // column job (0 = packed columns 0 and S)
     if (c < S) {
+        // loads and selects without branches (a branch per element would wait for
+        // each element's loads in turn); the parity of row y is the lane's (L is even);
+        // rows y >= s read in-bounds slots and are replaced by zero
+        const bool odd = lane & 1;
+        const int o2 = c == 0 ? S : M - c;
 #pragma unroll
-        for (int n1 = 0; n1 < R; ++n1) {
+        for (int n1 = 0; n1 < R / 2; ++n1) {
             const int y = L * n1 + lane;
-            if (n1 < R / 2 && y < s) {
+            const C* row = B + (y >> 1) * CS;
+            const C zk = row[c], zm = row[o2];
+            const C v = odd ? C{zk.im + zm.im, zm.re - zk.re} : C{zk.re + zm.re, zk.im - zm.im};
+            x[n1] = y < s ? v : C{T(0), T(0)};
+        }
+#pragma unroll
+        for (int n1 = R / 2; n1 < R; ++n1) x[n1] = C{T(0), T(0)};
+        if (c == 0) {
+            // packed real columns 0 and S (wave 0 only): Z[0] and Z[S] of the row pair
+#pragma unroll
+            for (int n1 = 0; n1 < R / 2; ++n1) {
+                const int y = L * n1 + lane;
                 const C* row = B + (y >> 1) * CS;
-                if (c == 0) {
-                    const C z0 = row[0], zs = row[S];
-                    x[n1] = (y & 1) ? C{z0.im, zs.im} : C{z0.re, zs.re};
-                } else {
-                    const C zk = row[c], zm = row[M - c];
-                    x[n1] = (y & 1) ? C{zk.im + zm.im, zm.re - zk.re} : C{zk.re + zm.re, zk.im - zm.im};
-                }
-            } else {
-                x[n1] = C{T(0), T(0)};
+                const C z0 = row[0], zs = row[S];
+                const C v = odd ? C{z0.im, zs.im} : C{z0.re, zs.re};
+                x[n1] = y < s ? v : C{T(0), T(0)};
             }
         }
     }
@@ -368,22 +378,30 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
         // with reference indices
         const uint32_t s2 = 2 * s;
         const int ya = 2 * q, yb = 2 * q + 1;
-        const bool in_a = ya <= s || ya > s + pad, in_b = yb <= s || yb > s + pad;
+        const bool in_a = !PAD || ya <= s || ya > s + pad, in_b = !PAD || yb <= s || yb > s + pad;
         const uint32_t ra = (uint32_t)(ya <= s ? ya : ya - pad) * s2, rb = (uint32_t)(yb <= s ? yb : yb - pad) * s2;
         T vmax = -(T)__builtin_huge_val();
         if (q < S && (in_a || in_b)) {
             C* slot = B + q * CS;
             fft_inv<T, R, L, M, true>(x, lane, slot, tw);
-            T* cout = a.corr_out ? a.corr_out + (size_t)sn * s2 * s2 : nullptr;
+            // the maximum through selects (no branch per element); positions outside
+            // the reference layout count as -inf (there are none when M = 2s)
+            const T lo = -(T)__builtin_huge_val();
 #pragma unroll
             for (int n1 = 0; n1 < R; ++n1) {
                 const int xx = L * n1 + lane;
                 slot[xx] = x[n1];
-                if (xx <= s || xx > s + pad) {
-                    const uint32_t rx = (uint32_t)(xx <= s ? xx : xx - pad);
-                    if (in_a) vmax = max_quiet(vmax, x[n1].re);
-                    if (in_b) vmax = max_quiet(vmax, x[n1].im);
-                    if (cout) {
+                const bool in_x = !PAD || xx <= s || xx > s + pad;
+                vmax = max_quiet(vmax, in_x && in_a ? x[n1].re : lo);
+                vmax = max_quiet(vmax, in_x && in_b ? x[n1].im : lo);
+            }
+            if (a.corr_out) {
+                T* cout = a.corr_out + (size_t)sn * s2 * s2;
+#pragma unroll
+                for (int n1 = 0; n1 < R; ++n1) {
+                    const int xx = L * n1 + lane;
+                    if (xx <= s || xx > s + pad) {
+                        const uint32_t rx = (uint32_t)(xx <= s ? xx : xx - pad);
                         if (in_a) cout[ra + rx] = x[n1].re;
                         if (in_b) cout[rb + rx] = x[n1].im;
                     }

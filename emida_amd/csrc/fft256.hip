@@ -329,7 +329,7 @@ __global__ __launch_bounds__(kNT, 1) void fft256_kernel(fft_args<float, IN, SUM>
         }
         if constexpr (PIPE) {
             asm volatile("" ::: "memory");
-            fft_inv<float, kR, kL, kM, true>(xs[j], lane, slot, tw2);
+            fft_inv<float, kR, kL, kM, true, true>(xs[j], lane, slot, tw2);
             if (j == kJ - 1) {
                 // job 3: column c = 96 + g, rows kL n1 + lane, to the LDS column store
                 C* col = B + kCol + g * kColStride;
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(kNT, 1) void fft256_kernel(fft_args<float, IN, SUM>
             C x[kR];
             row_pair_load(x, lane, band + 2 * g, kBS);
             __syncthreads();  // the band is read: its space becomes transpose scratch
-            fft_inv<float, kR, kL, kM, true>(x, lane, scr, tw2);
+            fft_inv<float, kR, kL, kM, true, true>(x, lane, scr, tw2);
             const int ya = y0 + 2 * g, yb = ya + 1;
             const bool in_a = ya <= s || ya > s + pad, in_b = yb <= s || yb > s + pad;
             const uint32_t ra = (uint32_t)(ya <= s ? ya : ya - pad) * s2;

@@ -190,19 +190,29 @@ __device__ __forceinline__ void fft_fwd(cplx<T> (&x)[R], int lane, cplx<T>* scr,
 
 // Inverse (unnormalised) transform from the frequency layout to natural order:
 // on return lane n2 holds output[L*n1 + n2] in x[n1].
-template <class T, int R, int L, int M, bool TW2D = false>
+// TW_FIRST: load each L-point DFT's twiddles before it.  tw and scr share LDS, so a
+// twiddle load written after a store to scr stays after it and each product waits for
+// its own load: hidden at four waves per SIMD (fft.hip), exposed at two (fft256.hip,
+// where loading first is 4% faster).
+template <class T, int R, int L, int M, bool TW2D = false, bool TW_FIRST = false>
 __device__ __forceinline__ void fft_inv(cplx<T> (&x)[R], int lane, cplx<T>* scr, const cplx<T>* tw) {
     constexpr int Q = R / L;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
+        const int k1 = lane + L * i;
+        cplx<T> w[L];
+        if constexpr (TW_FIRST) {
+#pragma unroll
+            for (int n2 = 1; n2 < L; ++n2) w[n2] = tw[tw_index<R, TW2D>(n2, k1)];
+        }
         cplx<T> y[L];
 #pragma unroll
         for (int k2 = 0; k2 < L; ++k2) y[k2] = x[i * L + k2];
         dft_reg<T, L, true>(y);
-        const int k1 = lane + L * i;
 #pragma unroll
         for (int n2 = 0; n2 < L; ++n2)
-            scr[n2 * (R + 1) + k1] = n2 == 0 ? y[0] : cmul_conj(y[n2], tw[tw_index<R, TW2D>(n2, k1)]);
+            scr[n2 * (R + 1) + k1] =
+                n2 == 0 ? y[0] : cmul_conj(y[n2], TW_FIRST ? w[n2] : tw[tw_index<R, TW2D>(n2, k1)]);
     }
     wave_sync();
 #pragma unroll

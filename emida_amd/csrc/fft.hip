@@ -161,7 +161,8 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
     constexpr int kLsqPer = (6 * 81 + NT - 1) / NT;  // least-squares elements per thread
 
     if (blockIdx.x >= a.total) return;
-    const uint32_t sn = xcd_slice(blockIdx.x, a.total, a.nb);
+    // workgroup-uniform (a scalar register: addresses built from it stay scalar)
+    const uint32_t sn = __builtin_amdgcn_readfirstlane(xcd_slice(blockIdx.x, a.total, a.nb));
     const uint32_t bn = sn % a.nb;
     const int tid = threadIdx.x;
     const int lane = tid % L, grp = tid / L;
@@ -175,17 +176,35 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
     // the first use (the window factors and the fit's least-squares rows too)
     const int kk = PIPE ? a.fit * a.fit : 0;
     if constexpr (FROM_PATTERN) {
-        const size2 b0 = a.begins[bn];
+        // the corner is workgroup-uniform: in scalar registers, every address below
+        // is a scalar base plus a lane offset
+        const size2 b0{(uint32_t)__builtin_amdgcn_readfirstlane(a.begins[bn].x),
+                       (uint32_t)__builtin_amdgcn_readfirstlane(a.begins[bn].y)};
         const IN* pic = a.pics + (size_t)(sn / a.nb) * a.src.area() + (size_t)b0.y * a.src.x + b0.x;
         constexpr int E = (S * S + NT - 1) / NT;
         IN px[E];
         T wx[E], wy[E];
+        if constexpr (!PAD && S == kWave) {
+            // element tid + it NT is row w + it NT/64 (w = the wave), column = lane:
+            // the row part of every address is wave-uniform (scalar registers)
+            const uint32_t w = __builtin_amdgcn_readfirstlane((uint32_t)tid / kWave), col = (uint32_t)tid % kWave;
+            const T hxv = a.hx[col];
 #pragma unroll
-        for (int it = 0; it < E; ++it) {
-            const int e = PAD ? min(tid + it * NT, s * s - 1) : tid + it * NT;  // clamped: masked below
-            px[it] = pic[(size_t)(e / s) * a.src.x + (e % s)];
-            wx[it] = a.hx[e % s];
-            wy[it] = a.hy[e / s];
+            for (int it = 0; it < E; ++it) {
+                const uint32_t row = w + it * (NT / kWave);
+                const IN* prow = pic + (size_t)row * a.src.x;  // uniform
+                px[it] = prow[col];
+                wx[it] = hxv;
+                wy[it] = a.hy[row];
+            }
+        } else {
+#pragma unroll
+            for (int it = 0; it < E; ++it) {
+                const int e = PAD ? min(tid + it * NT, s * s - 1) : tid + it * NT;  // clamped: masked below
+                px[it] = pic[(size_t)(e / s) * a.src.x + (e % s)];
+                wx[it] = a.hx[e % s];
+                wy[it] = a.hy[e / s];
+            }
         }
         double lv[kLsqPer];
         load_lsq<NT>(lv, a.lsq, kk, tid);
@@ -196,8 +215,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
             uint32_t su = 0;
 #pragma unroll
             for (int it = 0; it < E; ++it) su += tid + it * NT < s * s ? (uint32_t)px[it] : 0u;
-#pragma unroll
-            for (int off = kWave / 2; off > 0; off >>= 1) su += __shfl_xor(su, off);
+            su = wave_reduce(su, 0u, [](uint32_t x, uint32_t y) { return x + y; });
             if (tid % kWave == 0) red_u[tid / kWave] = su;
             __syncthreads();
             su = 0;

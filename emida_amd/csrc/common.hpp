@@ -103,6 +103,50 @@ __device__ inline void argmax_combine(T& v, uint32_t& i, T ov, uint32_t oi) {
     }
 }
 
+// ------------------------------------------------------------------ wave reductions
+// DPP lane move of a 32- or 64-bit value: lane i receives the value of the lane CTRL
+// selects; lanes in rows outside ROW_MASK receive `fill`.
+template <int CTRL, int ROW_MASK, class T>
+__device__ __forceinline__ T dpp_move(T fill, T v) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, fill), __builtin_bit_cast(int, v),
+                                                                 CTRL, ROW_MASK, 0xF, false));
+    } else {
+        static_assert(sizeof(T) == 8, "32- or 64-bit values");
+        const uint64_t f = __builtin_bit_cast(uint64_t, fill), x = __builtin_bit_cast(uint64_t, v);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)f, (int)(uint32_t)x, CTRL, ROW_MASK, 0xF, false);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)(f >> 32), (int)(uint32_t)(x >> 32), CTRL,
+                                                                  ROW_MASK, 0xF, false);
+        return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+    }
+}
+template <class T>
+__device__ __forceinline__ T lane63(T v) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+    } else {
+        const uint64_t x = __builtin_bit_cast(uint64_t, v);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, 63);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), 63);
+        return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+    }
+}
+// Reduction of a whole wave (all 64 lanes active) with an associative, commutative op
+// and its identity: quad swaps [1,0,3,2] and [2,3,0,1], half-row and row mirrors (every
+// lane of a 16-lane row then holds the row's result), row broadcasts 15 (rows 1, 3) and
+// 31 (rows 2, 3); lane 63 holds the wave's result, returned as a wave-uniform value.
+// Six DPP steps instead of six LDS permutes with their index arithmetic.
+template <class T, class OP>
+__device__ __forceinline__ T wave_reduce(T v, T ident, OP op) {
+    v = op(v, dpp_move<0xB1, 0xF>(ident, v));
+    v = op(v, dpp_move<0x4E, 0xF>(ident, v));
+    v = op(v, dpp_move<0x141, 0xF>(ident, v));
+    v = op(v, dpp_move<0x140, 0xF>(ident, v));
+    v = op(v, dpp_move<0x142, 0xA>(ident, v));
+    v = op(v, dpp_move<0x143, 0xC>(ident, v));
+    return lane63(v);
+}
+
 template <class T>
 __device__ inline void wave_argmax(T& v, uint32_t& i) {
 #pragma unroll
@@ -132,10 +176,10 @@ __device__ inline void block_argmax(T& v, uint32_t& i, T* sv, uint32_t* si) {
 }
 
 // Workgroup maximum (IEEE maxNum: NaN operands are ignored); every thread receives it.
+// Every thread of the workgroup calls it (the wave steps read all 64 lanes).
 template <class T, int NT>
 __device__ inline T block_max(T v, T* sv) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    v = wave_reduce(v, -(T)__builtin_huge_val(), [](T x, T y) { return fmax(x, y); });
     if (threadIdx.x % kWave == 0) sv[threadIdx.x / kWave] = v;
     __syncthreads();
     v = sv[0];
@@ -145,11 +189,10 @@ __device__ inline T block_max(T v, T* sv) {
     return v;
 }
 
-// Workgroup minimum of unsigned values; every thread receives it.
+// Workgroup minimum of unsigned values; every thread receives it (and calls it).
 template <int NT>
 __device__ inline uint32_t block_min_u32(uint32_t v, uint32_t* sv) {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off));
+    v = wave_reduce(v, 0xffffffffu, [](uint32_t x, uint32_t y) { return min(x, y); });
     if (threadIdx.x % kWave == 0) sv[threadIdx.x / kWave] = v;
     __syncthreads();
     v = sv[0];

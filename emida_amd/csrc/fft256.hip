@@ -201,10 +201,7 @@ __global__ __launch_bounds__(kNT, 1) void fft256_kernel(fft_args<float, IN, SUM>
                         const bool in = p < npairs && 2 * p + h < s && kL * n1 + lane < s;
                         su += in ? (uint32_t)raw[it][n1][h] : 0u;
                     }
-            // down-shuffles (lane 0 collects the wave's sum): their lane addresses differ
-            // from the arg-max's butterflies, so none is kept live across the kernel
-#pragma unroll
-            for (int off = kWave / 2; off > 0; off >>= 1) su += __shfl_down(su, off);
+            su = wave_reduce(su, 0u, [](uint32_t x, uint32_t y) { return x + y; });
             if (tid % kWave == 0) red_u[tid / kWave] = su;
             __syncthreads();
             su = 0;

@@ -221,50 +221,6 @@ __device__ __forceinline__ void fft_inv(cplx<T> (&x)[R], int lane, cplx<T>* scr,
     dft_reg<T, R, true>(x);
 }
 
-// ------------------------------------------------------------------ wave reductions
-// DPP lane move of a 32- or 64-bit value: lane i receives the value of the lane CTRL
-// selects; lanes in rows outside ROW_MASK receive `fill`.
-template <int CTRL, int ROW_MASK, class T>
-__device__ __forceinline__ T dpp_move(T fill, T v) {
-    if constexpr (sizeof(T) == 4) {
-        return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, fill), __builtin_bit_cast(int, v),
-                                                                 CTRL, ROW_MASK, 0xF, false));
-    } else {
-        static_assert(sizeof(T) == 8, "32- or 64-bit values");
-        const uint64_t f = __builtin_bit_cast(uint64_t, fill), x = __builtin_bit_cast(uint64_t, v);
-        const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)f, (int)(uint32_t)x, CTRL, ROW_MASK, 0xF, false);
-        const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)(f >> 32), (int)(uint32_t)(x >> 32), CTRL,
-                                                                  ROW_MASK, 0xF, false);
-        return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
-    }
-}
-template <class T>
-__device__ __forceinline__ T lane63(T v) {
-    if constexpr (sizeof(T) == 4) {
-        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
-    } else {
-        const uint64_t x = __builtin_bit_cast(uint64_t, v);
-        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, 63);
-        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), 63);
-        return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
-    }
-}
-// Reduction of a whole wave (all 64 lanes active) with an associative, commutative op
-// and its identity: quad swaps [1,0,3,2] and [2,3,0,1], half-row and row mirrors (every
-// lane of a 16-lane row then holds the row's result), row broadcasts 15 (rows 1, 3) and
-// 31 (rows 2, 3); lane 63 holds the wave's result, returned as a wave-uniform value.
-// Six DPP steps instead of six LDS permutes with their index arithmetic.
-template <class T, class OP>
-__device__ __forceinline__ T wave_reduce(T v, T ident, OP op) {
-    v = op(v, dpp_move<0xB1, 0xF>(ident, v));
-    v = op(v, dpp_move<0x4E, 0xF>(ident, v));
-    v = op(v, dpp_move<0x141, 0xF>(ident, v));
-    v = op(v, dpp_move<0x140, 0xF>(ident, v));
-    v = op(v, dpp_move<0x142, 0xA>(ident, v));
-    v = op(v, dpp_move<0x143, 0xC>(ident, v));
-    return lane63(v);
-}
-
 // Prepared pixel (src/prepare_pics.cu:36-41), no contraction.
 // Workgroup maximum / minimum for arrays used once per kernel: one barrier (the
 // trailing one that would guard the array's reuse is not needed).  Every thread of

@@ -167,8 +167,7 @@ __global__ __launch_bounds__(kGNT) void fftg_rows_kernel(fftg_args<T, IN, SUM> a
             uint32_t su = 0;
             for (int y = tid / 64; y < sy; y += kGNT / 64)
                 for (int x = tid % 64; x < sx; x += 64) su += pic[(size_t)y * a.src.x + x];
-#pragma unroll
-            for (int off = kWave / 2; off > 0; off >>= 1) su += __shfl_xor(su, off);
+            su = wave_reduce(su, 0u, [](uint32_t x, uint32_t y) { return x + y; });
             if (tid % kWave == 0) red_u[tid / kWave] = su;
             __syncthreads();
             su = 0;

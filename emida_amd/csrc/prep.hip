@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void sum_prepare_kernel(const uint16_t* __rest
             s += v[k];
         }
     }
-    for (int off = kWave / 2; off > 0; off >>= 1) s += (uint32_t)__shfl_xor((int)s, off);
+    s = wave_reduce(s, 0u, [](uint32_t x, uint32_t y) { return x + y; });
     if (tid % kWave == 0) part[tid / kWave] = s;
     __syncthreads();
     uint32_t sum = 0;

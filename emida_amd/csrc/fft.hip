@@ -240,6 +240,18 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
     }
     __syncthreads();
 
+    // this group's column of the reference spectrum (column job c = grp of stage B) is
+    // loaded in two halves, each well before its use: the first here, so that the row
+    // stage covers its latency, the second before the column transform
+    constexpr int PF = R / 2;
+    C spec[R];
+    if constexpr (PIPE) {
+        if (grp < S) {
+            const C* rc = a.spectra + ((size_t)bn * P::NC + grp) * M + lane * R;
+#pragma unroll
+            for (int r = 0; r < PF; ++r) spec[r] = rc[r];
+        }
+    }
     // ---- A. packed row pairs, forward (an odd s leaves the last pair's second row zero)
     if (grp < (s + 1) / 2) {
         const int p = grp;
@@ -295,15 +307,13 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
         }
     }
     __syncthreads();  // row spectra consumed; slots become column scratch
-    // the first half of this column's reference spectrum, loaded here so that the
-    // forward transform covers its latency (all of it: 2% slower, more registers)
-    constexpr int PF = R / 2;
-    C spec[R];
+    // the second half of the column's reference spectrum, covered by the forward
+    // transform (both halves here: 2% slower than the first half only)
     if constexpr (PIPE) {
         if (c < S) {
             const C* rc = a.spectra + ((size_t)bn * P::NC + c) * M + lane * R;
 #pragma unroll
-            for (int r = 0; r < PF; ++r) spec[r] = rc[r];
+            for (int r = PF; r < R; ++r) spec[r] = rc[r];
         }
     }
     if (c < S) {
@@ -327,7 +337,7 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
                     const C z = x[r], zm = slot[(M - k) & (M - 1)];
                     const C p0 = C{z.re + zm.re, z.im - zm.im}, ps = C{z.im + zm.im, zm.re - z.re};
                     if constexpr (PIPE) {
-                        const C h0 = hadamard(p0, r < PF ? spec[r] : a.spectra[c0 + r]), hs = hadamard(ps, a.spectra[cs + r]);
+                        const C h0 = hadamard(p0, spec[r]), hs = hadamard(ps, a.spectra[cs + r]);
                         x[r] = C{h0.re - hs.im, h0.im + hs.re};  // H0 + i HS
                     } else {
                         a.spectra_out[c0 + r] = C{p0.re * sc, p0.im * sc};
@@ -337,9 +347,8 @@ __global__ __launch_bounds__((fft_plan<T, M>::NT), (fft_plan<T, M>::WAVES_PER_SI
             wave_sync();
         } else {
             if constexpr (PIPE) {
-                const C* rc = a.spectra + ((size_t)bn * P::NC + c) * M + lane * R;
 #pragma unroll
-                for (int r = 0; r < R; ++r) x[r] = hadamard(x[r], r < PF ? spec[r] : rc[r]);
+                for (int r = 0; r < R; ++r) x[r] = hadamard(x[r], spec[r]);
             } else {
                 C* oc = a.spectra_out + ((size_t)bn * P::NC + c) * M + lane * R;
 #pragma unroll

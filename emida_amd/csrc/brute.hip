@@ -237,9 +237,16 @@ __global__ __launch_bounds__(256, 2) void cross_corr_tiled_kernel(const T* __res
     for (size_t e = tid; e * sizeof(T) < sh.lds; e += nt) ring[e] = T(0);
     __syncthreads();
 
+    // pattern row Y into its ring slot; rows past the slice (read by the last
+    // steps' y-shifts) are zeros, so the accumulation needs no per-row condition
+    // (rows before it only ever map to slots still holding the initial zeros)
     auto stage_prow = [&](int Y) {
-        if (Y < 0 || Y >= SY) return;
+        if (Y < 0) return;
         T* dst = ring + (size_t)(Y % ROWS) * SP * kTiledP;
+        if (Y >= SY) {
+            for (int e = tid; e < kTiledP * SP; e += nt) dst[e] = T(0);
+            return;
+        }
         for (int e = tid; e < kTiledP * S; e += nt) {
             const int p = e / S, k = e % S;
             const uint32_t pp = p0 + p;
@@ -286,8 +293,6 @@ __global__ __launch_bounds__(256, 2) void cross_corr_tiled_kernel(const T* __res
                     }
 #pragma unroll
                     for (int t = 0; t < kTiledTT; ++t) {
-                        const int Y = y + syt0 + t;
-                        if (Y < 0 || Y >= SY) continue;
 #pragma unroll
                         for (int n = 0; n < kTiledNI; ++n) {
                             const T b = (j - n >= 0) ? cur[(j - n) & 7] : prev[(8 + j - n) & 7];

@@ -199,8 +199,11 @@ struct pair_of {
     T v[2];
 };
 
-// Pattern ring: [rows][spad][16] with the 2-pattern pair index of column k stored at
-// pair ^ (k & 7) (conflict-free transposed writes and pair reads).
+// Pattern ring: [rows][spad][16], column k of a row holding the 16 patterns' values in
+// pattern order.  Staged with the pattern index fastest (consecutive threads write
+// consecutive words); a lane reads its pattern pair of column k8 + j at a per-row base
+// plus the compile-time offset 128 j bytes, so the inner loop needs no address
+// arithmetic per read (the 8 pairs a wave reads are 128 contiguous bytes).
 template <class T>
 __global__ __launch_bounds__(256, 2) void cross_corr_tiled_kernel(const T* __restrict__ pics,
                                                                    const T* __restrict__ ref, T* __restrict__ res,
@@ -248,10 +251,10 @@ __global__ __launch_bounds__(256, 2) void cross_corr_tiled_kernel(const T* __res
             return;
         }
         for (int e = tid; e < kTiledP * S; e += nt) {
-            const int p = e / S, k = e % S;
+            const int p = e % kTiledP, k = e / kTiledP;
             const uint32_t pp = p0 + p;
             const T v = pp < npat ? pics[((size_t)pp * nb + roi) * slice_elems + (size_t)Y * S + k] : T(0);
-            dst[k * kTiledP + 2 * ((p / 2) ^ (k & 7)) + (p & 1)] = v;
+            dst[k * kTiledP + p] = v;
         }
     };
     auto stage_rrow = [&](int y) {
@@ -274,6 +277,14 @@ __global__ __launch_bounds__(256, 2) void cross_corr_tiled_kernel(const T* __res
             const T* rp = rrow + sh.off - sxb;
 #pragma unroll
             for (int j = 0; j < 8; ++j) cur[j] = rp[klo - 8 + j];
+            // this lane's pattern pair in column klo of the rows of its 4 y-shifts
+            const pair_of<T>* pr[kTiledTT];
+#pragma unroll
+            for (int t = 0; t < kTiledTT; ++t) {
+                const int Y = y + syt0 + t;
+                const int slot = (Y % ROWS + ROWS) % ROWS;
+                pr[t] = reinterpret_cast<const pair_of<T>*>(ring + ((size_t)slot * SP + klo) * kTiledP) + pg;
+            }
             for (int k8 = klo; k8 < khi; k8 += 8) {
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
@@ -282,15 +293,9 @@ __global__ __launch_bounds__(256, 2) void cross_corr_tiled_kernel(const T* __res
                 }
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
-                    const int k = k8 + j;
                     pair_of<T> a[kTiledTT];
 #pragma unroll
-                    for (int t = 0; t < kTiledTT; ++t) {
-                        const int Y = y + syt0 + t;
-                        const int slot = (Y % ROWS + ROWS) % ROWS;
-                        a[t] = *reinterpret_cast<const pair_of<T>*>(ring + ((size_t)slot * SP + k) * kTiledP +
-                                                                    2 * (pg ^ (k & 7)));
-                    }
+                    for (int t = 0; t < kTiledTT; ++t) a[t] = pr[t][j * (kTiledP / 2)];
 #pragma unroll
                     for (int t = 0; t < kTiledTT; ++t) {
 #pragma unroll
@@ -301,6 +306,8 @@ __global__ __launch_bounds__(256, 2) void cross_corr_tiled_kernel(const T* __res
                         }
                     }
                 }
+#pragma unroll
+                for (int t = 0; t < kTiledTT; ++t) pr[t] += 8 * (kTiledP / 2);
             }
         }
         __syncthreads();

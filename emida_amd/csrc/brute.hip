@@ -272,11 +272,13 @@ __global__ __launch_bounds__(256, 2) void cross_corr_tiled_kernel(const T* __res
         stage_rrow(y);
         __syncthreads();
         if (klo < khi) {
-            // window values: cur[j] = R[y][k8 + j - sxb], prev[j] = R[y][k8 - 8 + j - sxb]
-            T prev[8], cur[8];
+            // window values: of the block being accumulated, cur[j] = R[y][k8 + j - sxb],
+            // and of the one before, prev[j] = R[y][k8 - 8 + j - sxb]; two register blocks
+            // alternate in the two roles (no copies), two column blocks per iteration
+            T wa[8], wb[8];
             const T* rp = rrow + sh.off - sxb;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) cur[j] = rp[klo - 8 + j];
+            for (int j = 0; j < 8; ++j) wa[j] = rp[klo - 8 + j];
             // this lane's pattern pair in column klo of the rows of its 4 y-shifts
             const pair_of<T>* pr[kTiledTT];
 #pragma unroll
@@ -285,12 +287,10 @@ __global__ __launch_bounds__(256, 2) void cross_corr_tiled_kernel(const T* __res
                 const int slot = (Y % ROWS + ROWS) % ROWS;
                 pr[t] = reinterpret_cast<const pair_of<T>*>(ring + ((size_t)slot * SP + klo) * kTiledP) + pg;
             }
-            for (int k8 = klo; k8 < khi; k8 += 8) {
+            // columns k8 .. k8 + 7: load the block into cur, accumulate
+            auto block = [&](int k8, const T (&prev)[8], T (&cur)[8]) {
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    prev[j] = cur[j];
-                    cur[j] = rp[k8 + j];
-                }
+                for (int j = 0; j < 8; ++j) cur[j] = rp[k8 + j];
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     pair_of<T> a[kTiledTT];
@@ -308,7 +308,13 @@ __global__ __launch_bounds__(256, 2) void cross_corr_tiled_kernel(const T* __res
                 }
 #pragma unroll
                 for (int t = 0; t < kTiledTT; ++t) pr[t] += 8 * (kTiledP / 2);
+            };
+            int k8 = klo;
+            for (; k8 + 8 < khi; k8 += 16) {
+                block(k8, wa, wb);
+                block(k8 + 8, wb, wa);
             }
+            if (k8 < khi) block(k8, wa, wb);
         }
         __syncthreads();
     }

@@ -156,7 +156,9 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
     HT* A = reinterpret_cast<HT*>(lds_raw);
     HT* Rs = reinterpret_cast<HT*>(lds_raw + P::lds_a);
 
-    const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+    // the wave index is wave-uniform: in a scalar register, the tile conditions below
+    // are scalar branches
+    const int tid = threadIdx.x, lane = tid % 64, wave = __builtin_amdgcn_readfirstlane(tid / 64);
     const uint32_t n_syt = div_up(cross.y, (uint32_t)ROWS);
     const uint32_t L = xcd_remap(blockIdx.x, n_wg);  // y-shift tiles of one (ROI, pattern tile) back to back
     const uint32_t syt = L % n_syt, slab = L / n_syt;
@@ -265,8 +267,12 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
             }
         }
     };
+    // pattern row Y into its ring slot; rows past the slice are zero rows (read by the
+    // last steps' y-shifts, so the matrix-core loop needs no per-row condition); rows
+    // before it map to slots still holding the initial zeros
     auto store_prow = [&](int Y) {
-        if (Y < 0 || Y >= SY) return;  // rows outside the slice are never read
+        if (Y < 0) return;
+        const bool zero = Y >= SY;
         const int slot = Y % ROWS;
 #pragma unroll
         for (int i = 0; i < P::PF; ++i) {
@@ -274,8 +280,8 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
             if (e < kMfmaM * CH) {
                 const int m = e / CH, c = e % CH;
                 const size_t o = ((size_t)slot * kMfmaM + m) * KMAX + 8 * P::swz(c, m);
-                *reinterpret_cast<V*>(A + o) = pfh[i];
-                if constexpr (SPLIT) *reinterpret_cast<V*>(A + P::ring + o) = pfl[i];
+                *reinterpret_cast<V*>(A + o) = zero ? V{} : pfh[i];
+                if constexpr (SPLIT) *reinterpret_cast<V*>(A + P::ring + o) = zero ? V{} : pfl[i];
             }
         }
     };
@@ -344,9 +350,14 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
             const int cs = 8 * P::swz(k0 / 8 + half, m_a);
 #pragma unroll
             for (int t = 0; t < TS; ++t) {
-                const int Y = y + syw + t;
-                if (Y < 0 || Y >= SY) continue;
-                const HT* arow = A + ((size_t)(Y % ROWS) * kMfmaM + m_a) * KMAX + cs;
+                const int Y = y + syw + t;  // >= -(ROWS - 1)
+                // rows outside the slice read zero rows; the split modes skip them (three
+                // products per row: the skip pays there, ~4% the other way in the plain
+                // modes, where the branch per row costs more than the products)
+                if constexpr (SPLIT) {
+                    if (Y < 0 || Y >= SY) continue;
+                }
+                const HT* arow = A + ((size_t)((Y + ROWS) % ROWS) * kMfmaM + m_a) * KMAX + cs;
                 const V ah = *reinterpret_cast<const V*>(arow);
                 V al{};
                 if constexpr (SPLIT) al = *reinterpret_cast<const V*>(arow + P::ring);

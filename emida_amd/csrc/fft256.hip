@@ -124,7 +124,8 @@ __device__ __forceinline__ void grow_row(cplx<float>* gr, const cplx<float> (&xs
 
 }  // namespace
 
-template <class IN, class SUM, bool FROM_PATTERN, bool PIPE>
+// FULL: the slice side is kS (a compile-time constant, every bound below folds)
+template <class IN, class SUM, bool FROM_PATTERN, bool PIPE, bool FULL>
 __global__ __launch_bounds__(kNT, 1) void fft256_kernel(fft_args<float, IN, SUM> a) {
     using C = cplx<float>;
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(kNT, 1) void fft256_kernel(fft_args<float, IN, SUM>
     const uint32_t sn = xcd_slice(blockIdx.x, a.total, a.nb);
     const uint32_t bn = sn % a.nb;
     const int tid = threadIdx.x, lane = tid % kL, g = tid / kL;
-    const int s = (int)a.side;  // slice side, s <= kS
+    const int s = FULL ? kS : (int)a.side;  // slice side, s <= kS
     // stored reference spectra: x 1/4 x (2s/256)^2 (the reference's (2s)^2 scale)
     const float sc = (float)(0.25 * ((double)(2 * s) / kM) * ((double)(2 * s) / kM));
 
@@ -526,15 +527,21 @@ __global__ __launch_bounds__(kNT, 1) void fft256_kernel(fft_args<float, IN, SUM>
 }
 
 // ---------------------------------------------------------------- launcher
-template <class IN, class SUM, bool FP, bool PIPE>
-void launch_fft256(fft_args<float, IN, SUM> a, hipStream_t st) {
-    auto k = fft256_kernel<IN, SUM, FP, PIPE>;
+template <class IN, class SUM, bool FP, bool PIPE, bool FULL>
+static void launch_fft256_s(fft_args<float, IN, SUM> a, hipStream_t st) {
+    auto k = fft256_kernel<IN, SUM, FP, PIPE, FULL>;
     static const hipError_t attr =
         hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLds);
     EMIDA_HIP(attr);
-    if (a.total == 0) return;
     k<<<a.total, kNT, kLds, st>>>(a);
     EMIDA_LAUNCHED("fft256_kernel");
+}
+
+template <class IN, class SUM, bool FP, bool PIPE>
+void launch_fft256(fft_args<float, IN, SUM> a, hipStream_t st) {
+    if (a.total == 0) return;
+    if (a.side == (uint32_t)kS) launch_fft256_s<IN, SUM, FP, PIPE, true>(a, st);
+    else launch_fft256_s<IN, SUM, FP, PIPE, false>(a, st);
 }
 
 #define INST_FFT256(IN, SUM)                                                                        \

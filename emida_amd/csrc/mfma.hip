@@ -338,39 +338,49 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
                 any = any || nz[u];
             }
             if (!any) continue;
-            V bh[kMfmaTilesPerWave], bl[kMfmaTilesPerWave];
-#pragma unroll
-            for (int u = 0; u < kMfmaTilesPerWave; ++u) {
-                if (!nz[u]) continue;
-                const int jp = k0 + 8 * half - sx0[u] - m_a + off;
-                const int rho = jp & 7;
-                bh[u] = *reinterpret_cast<const V*>(Rs + rho * P::RP + (jp - rho));
-                if constexpr (SPLIT) bl[u] = *reinterpret_cast<const V*>(Rs + (8 + rho) * P::RP + (jp - rho));
-            }
             const int cs = 8 * P::swz(k0 / 8 + half, m_a);
-#pragma unroll
-            for (int t = 0; t < TS; ++t) {
-                const int Y = y + syw + t;  // >= -(ROWS - 1)
-                // rows outside the slice read zero rows; the split modes skip them (three
-                // products per row: the skip pays there, ~4% the other way in the plain
-                // modes, where the branch per row costs more than the products)
-                if constexpr (SPLIT) {
-                    if (Y < 0 || Y >= SY) continue;
-                }
-                const HT* arow = A + ((size_t)((Y + ROWS) % ROWS) * kMfmaM + m_a) * KMAX + cs;
-                const V ah = *reinterpret_cast<const V*>(arow);
-                V al{};
-                if constexpr (SPLIT) al = *reinterpret_cast<const V*>(arow + P::ring);
+            // the step for a compile-time set of non-zero tiles (U0, U1): straight-line
+            // loads and products, the tile conditions tested once per step
+            auto step = [&](auto u0, auto u1) {
+                constexpr bool U[2] = {decltype(u0)::value, decltype(u1)::value};
+                V bh[kMfmaTilesPerWave], bl[kMfmaTilesPerWave];
 #pragma unroll
                 for (int u = 0; u < kMfmaTilesPerWave; ++u) {
-                    if (!nz[u]) continue;
-                    acc[t][u] = mfma_vec<HT>::mma(ah, bh[u], acc[t][u]);
+                    if (!U[u]) continue;
+                    const int jp = k0 + 8 * half - sx0[u] - m_a + off;
+                    const int rho = jp & 7;
+                    bh[u] = *reinterpret_cast<const V*>(Rs + rho * P::RP + (jp - rho));
+                    if constexpr (SPLIT) bl[u] = *reinterpret_cast<const V*>(Rs + (8 + rho) * P::RP + (jp - rho));
+                }
+#pragma unroll
+                for (int t = 0; t < TS; ++t) {
+                    const int Y = y + syw + t;  // >= -(ROWS - 1)
+                    // rows outside the slice read zero rows; the split modes skip them
+                    // (three products per row: the skip pays there, ~4% the other way in
+                    // the plain modes, where the branch per row costs more than the products)
                     if constexpr (SPLIT) {
-                        acc[t][u] = mfma_vec<HT>::mma(ah, bl[u], acc[t][u]);
-                        acc[t][u] = mfma_vec<HT>::mma(al, bh[u], acc[t][u]);
+                        if (Y < 0 || Y >= SY) continue;
+                    }
+                    const HT* arow = A + ((size_t)((Y + ROWS) % ROWS) * kMfmaM + m_a) * KMAX + cs;
+                    const V ah = *reinterpret_cast<const V*>(arow);
+                    V al{};
+                    if constexpr (SPLIT) al = *reinterpret_cast<const V*>(arow + P::ring);
+#pragma unroll
+                    for (int u = 0; u < kMfmaTilesPerWave; ++u) {
+                        if (!U[u]) continue;
+                        acc[t][u] = mfma_vec<HT>::mma(ah, bh[u], acc[t][u]);
+                        if constexpr (SPLIT) {
+                            acc[t][u] = mfma_vec<HT>::mma(ah, bl[u], acc[t][u]);
+                            acc[t][u] = mfma_vec<HT>::mma(al, bh[u], acc[t][u]);
+                        }
                     }
                 }
-            }
+            };
+            static_assert(kMfmaTilesPerWave == 2, "two tiles per wave");
+            using T1 = std::true_type;
+            using F1 = std::false_type;
+            if (nz[0]) step(T1{}, F1{});
+            if (nz[1]) step(F1{}, T1{});
         }
         // the next row's halves while this step's matrix-core work drains (converting
         // them halfway through the step costs 4 spilled registers at 128 columns)

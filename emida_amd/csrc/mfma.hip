@@ -137,7 +137,7 @@ __device__ inline void prep_chunk(const uint32_t (&u)[4], float mean, const floa
 
 // FP: the pattern rows are read as uint16 pixels and prepared while they are staged
 // (phi/plo unused); otherwise phi/plo hold the prepared halves.
-template <class HT, bool SPLIT, int KMAX, int WX, bool FP>
+template <class HT, bool SPLIT, int KMAX, int WX, bool FP, bool FULL>
 __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restrict__ phi,
                                                                 const HT* __restrict__ plo,
                                                                 const HT* __restrict__ rhi,
@@ -166,7 +166,8 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
     const int rx = (int)(cross.x - 1) / 2, ry = (int)(cross.y - 1) / 2;
     const int sy0 = (int)syt * ROWS - ry;  // first y-shift of the workgroup
     const int off = P::SXW - rx;            // shifted-copy offset, see below
-    const int S = (int)s.x, SY = (int)s.y, CH = S / 8;
+    // FULL: KMAX x KMAX slices (compile-time sizes, a fixed number of K steps)
+    const int S = FULL ? KMAX : (int)s.x, SY = FULL ? KMAX : (int)s.y, CH = S / 8;
     const int K16 = (S + 15) / 16 * 16;
     const uint32_t p0 = mt * kMfmaM;
     const size_t slice_elems = (size_t)s.x * s.y;
@@ -527,10 +528,16 @@ void launch_mfma_corr_g(const HT* phi, const HT* plo, const HT* rhi, const HT* r
     using P = mfma_plan<HT, SPLIT, KMAX, WX>;
     static_assert(P::lds_bytes <= 80 * 1024, "two workgroups per CU");
     static_assert((size_t)P::ROWS * kMfmaM * 16 <= P::lds_bytes, "epilogue row tables exceed the LDS");
-    auto kern = mfma_corr_kernel<HT, SPLIT, KMAX, WX, FP>;
+    const bool full = s.x == (uint32_t)KMAX && s.y == (uint32_t)KMAX;
+    auto kern = full ? mfma_corr_kernel<HT, SPLIT, KMAX, WX, FP, true> : mfma_corr_kernel<HT, SPLIT, KMAX, WX, FP, false>;
     static const hipError_t attr =
-        hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P::lds_bytes);
+        hipFuncSetAttribute((const void*)mfma_corr_kernel<HT, SPLIT, KMAX, WX, FP, false>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)P::lds_bytes);
+    static const hipError_t attr_full =
+        hipFuncSetAttribute((const void*)mfma_corr_kernel<HT, SPLIT, KMAX, WX, FP, true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)P::lds_bytes);
     EMIDA_HIP(attr);
+    EMIDA_HIP(attr_full);
     if (s.x > (uint32_t)KMAX || cross.x > (uint32_t)P::SXW) throw arg_error("matrix-core geometry mismatch");
     const uint64_t n_wg = (uint64_t)nb * div_up(npat, (uint32_t)kMfmaM) * div_up(cross.y, (uint32_t)P::ROWS);
     if (n_wg > (1u << 24)) throw unsupported_error("matrix-core brute policy: too many workgroups in one launch");

@@ -107,20 +107,28 @@ def test_alone_equals_plain_pipeline(workload):
 @pytest.mark.parametrize("use_graphs", [True, False])
 def test_mix_equals_alone(workload, use_graphs):
     ps, plans = workload
-    srv, ids = _server(ps, plans, use_graphs=use_graphs, max_batch=8, streams=4)
+    # one batch slot per plan: requests for a plan that arrive while its batch runs queue
+    # behind it and launch together
+    srv, ids = _server(ps, plans, use_graphs=use_graphs, max_batch=8, streams=4, slots_per_plan=1)
     with srv:
-        # alone first, then two mixed rounds (graph replays with eager work between)
+        # alone first, then mixed rounds (graph replays with eager work between): at least
+        # two, and up to four until some requests were batched together (whether they are
+        # depends on how fast the submitting threads run against the device)
         alone = {}
         for pi in range(len(plans)):
             rid, offs, coefs = srv.submit(ids[pi], ps.pics[0])
             srv.wait(rid)
             alone[(pi, 0)] = (offs, coefs)
         _check(plans, alone)
-        for r in range(2):
+        batched = False
+        for r in range(4):
             latest, allres, infos = _mix(srv, ids, ps, plans, rounds=2, seed=r)
             for (pi, i, _), v in allres.items():
                 _check(plans, {(pi, i): v})
-            assert max(inf["batch"] for inf in infos) > 1  # requests were batched together
+            batched = batched or max(inf["batch"] for inf in infos) > 1
+            if r >= 1 and batched:
+                break
+        assert batched  # requests were batched together
         st = srv.stats()
         assert st["completed"] == st["submitted"]
         assert st["launches"] < st["submitted"]

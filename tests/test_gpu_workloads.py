@@ -77,7 +77,9 @@ def served():
                 plain.append((offs, coefs, maxes.copy()))
         plans.append(dict(r=r, k=k, T=T, pol=pol, mode=mode, begins=begins, cross=cross, plain=plain))
 
-    srv = emida_amd.Server(hbm_budget=8 << 30, max_batch=8, streams=4, max_queued=256)
+    # one batch slot per plan: a plan's requests that arrive while its batch runs queue
+    # behind it and share the next launch
+    srv = emida_amd.Server(hbm_budget=8 << 30, max_batch=8, streams=4, max_queued=256, slots_per_plan=1)
     with srv:
         for p in plans:
             p["pid"] = srv.add_plan(p["T"], ps.ref, p["begins"], (2 * p["r"], 2 * p["r"]), p["cross"], p["pol"], FIT,

@@ -357,9 +357,10 @@ __global__ __launch_bounds__(kMfmaNT, 2) void mfma_corr_kernel(const HT* __restr
                 for (int t = 0; t < TS; ++t) {
                     const int Y = y + syw + t;  // >= -(ROWS - 1)
                     // rows outside the slice read zero rows; the split modes skip them
-                    // (three products per row: the skip pays there, ~4% the other way in
-                    // the plain modes, where the branch per row costs more than the products)
-                    if constexpr (SPLIT) {
+                    // unless the K loop is unrolled (three products per row: the skip pays
+                    // there, ~4% the other way in the plain modes and 2% in the unrolled
+                    // split loop, where the branch per row costs more than the products)
+                    if constexpr (SPLIT && !FULL) {
                         if (Y < 0 || Y >= SY) continue;
                     }
                     const HT* arow = A + ((size_t)((Y + ROWS) % ROWS) * kMfmaM + m_a) * KMAX + cs;
